@@ -1,0 +1,323 @@
+/* ssolh -- thin C-ABI between the host library (libssol) and the gfx950 HIP
+ * kernels (libssol_hip).
+ *
+ * The reference has no device boundary: ssol_solve runs its OpenMP
+ * realisation loop (src/ssol_solver.c:703-732) straight over Star-3D scene
+ * views. In this build ssol_solve flattens the scene into the plain arrays
+ * below (host memory, fixed-width types, no C++ or torch types), hands them
+ * to ssolh_scene_upload, runs ssolh_solve, and reads the tallies back with
+ * ssolh_tallies_download. Every entry point returns 0 on success, a negative
+ * HIP error or one of the SSOLH_ERR_* codes otherwise; ssolh_last_error
+ * gives the message.
+ *
+ * Conventions
+ *  - 3x4 transforms are column major like the reference's: element (row r,
+ *    column c) of the linear part is m[c*3+r], the translation is m[9..11].
+ *  - Triangle front side: the geometric normal is (v2-v0) x (v1-v0), the
+ *    left-handed convention of Star-3D hits (front = clockwise vertices).
+ *  - Tally slots hold {sum of weights, sum of squared weights} pairs, each
+ *    realisation's contributions summed before squaring
+ *    (reference src/ssol_estimator_c.h:66-88).
+ */
+#ifndef SSOLH_H
+#define SSOLH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSOLH_API extern __attribute__((visibility("default")))
+
+enum ssolh_error {
+  SSOLH_OK = 0,
+  SSOLH_ERR_ARG = 1,       /* invalid descriptor */
+  SSOLH_ERR_NO_DEVICE = 2, /* no HIP device / wrong architecture */
+  SSOLH_ERR_MEM = 3,       /* device allocation failed */
+  SSOLH_ERR_LIMIT = 4      /* a fixed capacity of the kernels was exceeded */
+};
+
+enum ssolh_shape_type { SSOLH_SHAPE_MESH = 0, SSOLH_SHAPE_PUNCHED = 1 };
+
+/* Same numbering as enum ssol_quadric_type */
+enum ssolh_quadric {
+  SSOLH_Q_PLANE = 0, SSOLH_Q_PARABOL, SSOLH_Q_PARABOL2F, SSOLH_Q_HYPERBOL,
+  SSOLH_Q_PARABOLIC_CYLINDER, SSOLH_Q_HEMISPHERE
+};
+
+/* Same numbering as enum ssol_material_type */
+enum ssolh_material_type {
+  SSOLH_MTL_DIELECTRIC = 0, SSOLH_MTL_MATTE, SSOLH_MTL_MIRROR,
+  SSOLH_MTL_THIN_DIELECTRIC, SSOLH_MTL_VIRTUAL
+};
+
+enum ssolh_sun_type {
+  SSOLH_SUN_DIRECTIONAL = 0, SSOLH_SUN_PILLBOX, SSOLH_SUN_GAUSSIAN,
+  SSOLH_SUN_BUIE
+};
+
+/* A value that is a constant or a spectrum of the scene's spectrum pool */
+struct ssolh_data {
+  double real;        /* used when spectrum < 0 */
+  int32_t spectrum;   /* index into ssolh_scene.spectra, or -1 */
+  int32_t pad__;
+};
+
+struct ssolh_spectrum {
+  uint32_t offset;    /* into spectra_wl / spectra_val */
+  uint32_t count;
+};
+
+/* Media are deduplicated by the host with the reference's media_ceq rule
+ * (src/ssol_material.c:730-736): equal ids <=> media_ceq */
+struct ssolh_medium {
+  struct ssolh_data extinction;
+  struct ssolh_data refractive_index;
+};
+
+/* Material, with its shaders already evaluated (baked) by the host */
+struct ssolh_material {
+  int32_t type;             /* enum ssolh_material_type */
+  int32_t distribution;     /* mirror: 0 Beckmann, 1 pillbox */
+  int32_t medium_in;        /* dielectric / thin dielectric media ids */
+  int32_t medium_out;
+  int32_t normal_from_ng;   /* 1: shader returns Ng, 0: returns Ns */
+  int32_t pad__;
+  double reflectivity;      /* mirror, matte */
+  double roughness;         /* mirror; 0 <=> specular */
+  double thickness;         /* thin dielectric */
+  struct ssolh_data slab_eta;        /* thin dielectric slab index */
+  struct ssolh_data slab_absorption; /* thin dielectric slab extinction */
+};
+
+/* Ray-tracing shape. Mesh shapes and punched surfaces share the tessellated
+ * triangle representation (punched: the lifted quadric mesh, reference
+ * src/ssol_shape.c:694-763); punched shapes add the analytic quadric used by
+ * the hit refinement (src/ssol_shape.c:1449-1494). */
+struct ssolh_shape {
+  int32_t type;             /* enum ssolh_shape_type */
+  int32_t quadric;          /* enum ssolh_quadric, punched only */
+  uint32_t ntris;
+  uint32_t pad__;
+  double qparam[4];         /* private quadric data, see below */
+  double qtransform[12];    /* quadric -> object space */
+};
+/* qparam by quadric:
+ *  PARABOL:   {focal, 1/(4 focal)}
+ *  PARABOL2F: {1/(4 fx), 1/(4 fy)}
+ *  HYPERBOL:  {g/2, a2/b2, 1/a2, |b|} (reference priv_hyperbol_data)
+ *  PARABOLIC_CYLINDER: {focal, 1/(4 focal)}
+ *  HEMISPHERE: {radius, radius2} */
+
+/* One shaded shape of an object */
+struct ssolh_sshape {
+  uint32_t shape;
+  uint32_t mtl_front;
+  uint32_t mtl_back;
+  uint32_t pad__;
+};
+
+/* Object = BLAS over the triangles of its shaded shapes */
+struct ssolh_object {
+  uint32_t sshape_first;    /* into sshapes */
+  uint32_t sshape_count;
+  uint32_t blas_root;       /* into nodes */
+  uint32_t pad__;
+};
+
+/* BVH node (BVH2, both child boxes stored in the parent). A child with
+ * count == 0 is the inner node `child`; count == SSOLH_BVH_EMPTY marks an
+ * absent child; otherwise it is a leaf over [child, child+count) of the leaf
+ * primitives (triangles for a BLAS, TLAS leaf slots for the TLAS). */
+#define SSOLH_BVH_EMPTY 0xFFFFFFFFu
+struct ssolh_bvh_node {
+  float lo[2][3];
+  float hi[2][3];
+  uint32_t child[2];
+  uint32_t count[2];
+};
+
+/* BLAS triangle, object space, reference winding */
+struct ssolh_triangle {
+  float v0[3];
+  float v1[3];
+  float v2[3];
+  uint32_t sshape;          /* shaded-shape index local to its object */
+  uint32_t prim;            /* triangle index within its shape */
+  uint32_t pad__;
+};
+
+/* Instance of an object */
+struct ssolh_instance {
+  double transform[12];     /* object -> world */
+  double inv_linear[9];     /* inverse of the linear part, column major */
+  uint32_t object;
+  uint32_t receiver_mask;   /* ssol_side_flag bits */
+  int32_t receiver;         /* receiver tally slot or -1 */
+  int32_t sampled;          /* sampled tally slot or -1 */
+  uint32_t isshape_first;   /* into inst_sshapes: one per shaded shape */
+  uint32_t user_id;
+};
+
+/* Per (instance, shaded shape): world <-> quadric frame for punched shapes
+ * and the per-primitive tally base (reference src/ssol_shape.c:1461-1466,
+ * composite of instance and quadric transforms) */
+struct ssolh_inst_sshape {
+  double R[9];              /* quadric -> world linear part */
+  double T[3];              /* quadric -> world translation */
+  double R_inv[9];          /* world -> quadric linear part */
+  int64_t prim_tally;       /* first per-primitive slot or -1 */
+};
+
+/* Sampled triangle of the sampling view (flat proxy for punched shapes,
+ * reference src/ssol_shape.c:767-811), object space */
+struct ssolh_sample_tri {
+  float v0[3];
+  float v1[3];
+  float v2[3];
+  uint32_t instance;
+  uint32_t sshape;          /* local to the instance's object */
+  uint32_t prim;
+};
+
+/* Alias table entry over the sampled triangles, area weighted */
+struct ssolh_alias {
+  double prob;
+  uint32_t alias;
+  uint32_t pad__;
+};
+
+struct ssolh_sun {
+  int32_t type;             /* enum ssolh_sun_type */
+  int32_t wl_spectrum;      /* sun spectrum index in the pool */
+  double dir[3];            /* normalised main direction */
+  double basis[9];          /* column major frame whose z column is dir */
+  double dni;
+  double param[8];          /* pillbox: sin2 theta_max; gaussian: sigma;
+                               buie: thetaSD, dThetaCSSD, gamma,
+                               e^k 1000^gamma, alpha, h1, h2, p_rect1 */
+  double wl_cdf_total;      /* piecewise-linear wavelength CDF normaliser */
+};
+
+struct ssolh_scene {
+  uint32_t nspectra, spectra_len;
+  const struct ssolh_spectrum* spectra;
+  const double* spectra_wl;
+  const double* spectra_val;
+  const double* spectra_cdf; /* cumulative trapezoid areas, per spectrum */
+
+  uint32_t nmedia, air_medium;
+  const struct ssolh_medium* media;
+
+  uint32_t nmaterials;
+  const struct ssolh_material* materials;
+
+  uint32_t nshapes;
+  const struct ssolh_shape* shapes;
+
+  uint32_t nsshapes;
+  const struct ssolh_sshape* sshapes;
+
+  uint32_t nobjects;
+  const struct ssolh_object* objects;
+
+  uint32_t nnodes;
+  const struct ssolh_bvh_node* nodes;   /* BLASes then TLAS */
+  uint32_t tlas_root;
+
+  uint32_t ntriangles;
+  const struct ssolh_triangle* triangles;
+
+  uint32_t ninstances;
+  const struct ssolh_instance* instances;
+  const uint32_t* tlas_instances;       /* TLAS leaf order -> instance */
+
+  uint32_t ninst_sshapes;
+  const struct ssolh_inst_sshape* inst_sshapes;
+
+  uint32_t nsample_tris;
+  const struct ssolh_sample_tri* sample_tris;
+  const struct ssolh_alias* sample_alias;
+
+  struct ssolh_sun sun;
+  double sampled_area;        /* S: sum of RT areas of sampled instances */
+  double sampled_area_proxy;  /* S_proxy: sum of sampling-mesh areas */
+
+  /* Tally layout */
+  uint32_t nreceivers;        /* receiver slots (2 sides each) */
+  uint32_t nsampled;          /* sampled-instance slots */
+  uint64_t nprim_tallies;     /* per-primitive slots (2 sides each) */
+};
+
+enum ssolh_precision { SSOLH_FP64 = 0, SSOLH_FP32 = 1 };
+
+struct ssolh_solve_args {
+  uint64_t realisations;
+  uint64_t max_failed;        /* reference max_failed_count */
+  uint64_t rng_key;           /* Threefry key word 0 */
+  uint64_t rng_counter;       /* first realisation counter */
+  uint32_t precision;         /* enum ssolh_precision */
+  uint32_t path_slots;        /* 0 <=> automatic */
+  uint32_t pending_entries;   /* per-path pending tally entries, 0 => 4 */
+  uint32_t pad__;
+};
+
+/* Number of doubles of each tally array (pairs {sum w, sum w2}) */
+#define SSOLH_GLOBAL_FIELDS 6   /* order: cos, absorbed_by_receivers,
+                                   shadowed, missing, extinguished,
+                                   other_absorbed */
+#define SSOLH_RCV_FIELDS 10     /* ssol_mc_receiver field order */
+
+struct ssolh_tallies {
+  double global[SSOLH_GLOBAL_FIELDS][2];
+  double* receivers;          /* [nreceivers][2 sides][10][2] */
+  double* sampled;            /* [nsampled][2: cos, shadowed][2] */
+  uint64_t* sampled_count;    /* [nsampled] nb_samples */
+  double* sampled_x_receivers;/* [nsampled][nreceivers][2][10][2] */
+  double* primitives;         /* [nprim_tallies][2 sides][10][2] */
+  uint64_t realisation_count; /* successful realisations */
+  uint64_t failed_count;
+  uint64_t segments;          /* traced segments incl. shadow rays */
+  uint64_t pending_spills;
+  uint32_t aborted;           /* failures reached max_failed */
+  uint32_t iterations;
+  double kernel_ms, trace_ms, shade_ms;
+};
+
+typedef struct ssolh_context ssolh_context;
+
+SSOLH_API int ssolh_create(int device, ssolh_context** ctx);
+SSOLH_API int ssolh_destroy(ssolh_context* ctx);
+SSOLH_API const char* ssolh_last_error(void);
+SSOLH_API int ssolh_device_count(void);
+
+/* Copy the flat scene to the device (replaces the previous one) */
+SSOLH_API int ssolh_scene_upload(ssolh_context* ctx, const struct ssolh_scene* scn);
+
+/* Run the realisations of one solve over the uploaded scene */
+SSOLH_API int ssolh_solve(ssolh_context* ctx, const struct ssolh_solve_args* args);
+
+/* Copy the tallies of the last solve into host arrays the caller sized
+ * from the uploaded scene's tally layout */
+SSOLH_API int ssolh_tallies_download(ssolh_context* ctx, struct ssolh_tallies* out);
+
+/* Deterministic kernels, one launch over n inputs each, for known-answer
+ * tests against the oracle. Inputs and outputs are host arrays. */
+SSOLH_API int ssolh_test_quadric_hits
+  (ssolh_context* ctx, int quadric, const double qparam[4], uint32_t n,
+   const double* org, const double* dir, const double* hint,
+   double* t, double* normal);
+SSOLH_API int ssolh_test_sun_dirs
+  (ssolh_context* ctx, const struct ssolh_sun* sun, uint32_t n,
+   const double* u, /* [n][2] */ double* dir);
+SSOLH_API int ssolh_test_threefry
+  (ssolh_context* ctx, uint32_t n, const uint64_t* ctr, const uint64_t* key,
+   uint64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSOLH_H */

@@ -1,0 +1,1327 @@
+// ssolh -- wavefront flux integrator on one MI355X (gfx950).
+//
+// Replaces the reference's OpenMP realisation loop (src/ssol_solver.c:703-732
+// running trace_radiative_path, :873-1127) by a persistent wavefront of path
+// slots in HBM:
+//
+//   k_generate  fills the slots with new realisations (start point, sun
+//               direction, wavelength, weights) and queues their shadow rays;
+//   k_trace     one thread per queued ray: two-level BVH traversal in FP32
+//               boxes with the reference's any-hit filter inline
+//               (src/ssol_scene.c:438-536) and the FP64 quadric refinement
+//               (src/ssol_shape.c:1449-1494);
+//   k_shade     one thread per ray result: the tail of the reference loop
+//               iteration that traced it (miss, segment extinction tallies,
+//               Russian roulette, point update) and the head of the next one
+//               (extinction, BSDF or virtual pass-through, receiver tallies);
+//               finished realisations flush their pending tallies with
+//               wave-aggregated FP64 atomics and the slot is refilled with a
+//               new realisation in place; live rays are compacted into the
+//               next queue with one atomic per wave.
+//
+// The host loop alternates k_trace / k_shade until the queue is empty.
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "ssolh.h"
+#include "ssolh_device.hpp"
+
+using namespace ssolh;
+
+namespace {
+
+constexpr int BLOCK = 128;           // threads per block (2 waves)
+constexpr int STACK = 48;            // traversal stack entries per thread (LDS)
+constexpr uint32_t SENTINEL = 0xFFFFFFFFu;
+constexpr uint32_t LEAF = 0x80000000u;
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr int MAX_PEND = 8;          // compile-time bound of pending entries
+
+// Per-thread message of the last failing call
+thread_local char g_err[512] = "";
+
+void set_err(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t e__ = (expr);                                                 \
+    if (e__ != hipSuccess) {                                                 \
+      set_err("%s: %s (%s:%d)", #expr, hipGetErrorString(e__), __FILE__,     \
+              __LINE__);                                                     \
+      return -(int)e__;                                                      \
+    }                                                                        \
+  } while (0)
+
+// -------------------------------------------------------------------------
+// Device scene
+// -------------------------------------------------------------------------
+struct DScene {
+  Spectra spectra;
+  const ssolh_medium* media;
+  uint32_t air;
+  const ssolh_material* mtls;
+  const ssolh_shape* shapes;
+  const ssolh_sshape* sshapes;
+  const ssolh_object* objects;
+  const ssolh_bvh_node* nodes;
+  uint32_t tlas_root;
+  const ssolh_triangle* tris;
+  const ssolh_instance* insts;
+  const uint32_t* tlas_inst;
+  const ssolh_inst_sshape* isshapes;
+  const ssolh_sample_tri* stris;
+  const ssolh_alias* alias;
+  uint32_t nstris;
+  ssolh_sun sun;
+  double S, S_proxy;
+  uint32_t nrcv, nsamp;
+};
+
+// Ray and hit records, indexed by queue position --------------------------
+enum : uint32_t {
+  RF_SIDE_SHIFT = 16,     // 2 bits: side of the origin point
+  RF_DISCARD_VIRTUAL = 1u << 18,
+  RF_REVERSED = 1u << 19,
+  RF_ANY_HIT = 1u << 20
+};
+
+struct alignas(16) RayRec {
+  double ox, oy, oz, tmin;
+  double dx, dy, dz, pad;
+  uint32_t slot, inst_from, prim_from, flags;  // flags: sshape_from | RF_*
+};
+
+struct alignas(16) HitRec {
+  double t, qdst;
+  double qnx, qny, qnz;
+  float ngx, ngy, ngz;
+  uint32_t inst, prim, sshape;  // inst == NONE <=> miss
+};
+
+// Path state, indexed by slot ---------------------------------------------
+enum : uint32_t {
+  PF_STAGE_BOUNCE = 1u << 0,   // else: waiting for the shadow ray
+  PF_HIT_VIRTUAL = 1u << 1,    // the point that emitted the ray is virtual
+  PF_IN_ATM = 1u << 2,         // the incoming segment of that point was in air
+  PF_ACTIVE = 1u << 3
+};
+
+struct PathState {
+  double4* pos_wl;     // pos, wavelength
+  double4* dir_cos;    // direction, cos factor
+  double4* prev_init;  // previous outgoing weights (3), initial flux
+  double4* out_in0;    // outgoing weights of the last shaded point (3), its incoming flux
+  double4* nrm_t;      // normal of the current point, last hit distance
+  double4* pend;       // pending absorbed_by_receivers, extinguished, other
+  uint4* meta;         // path id lo/hi, rng block, flags
+  uint4* cur;          // inst, prim, sshape | side << 16, sampled slot
+  uint4* misc;         // depth, in medium | out medium << 16, survivor score, roulette interval
+  uint32_t* pkey;      // [npend][slots]: receiver key (slot*2+side) or NONE
+  double* pval;        // [npend][10][slots]
+  uint32_t* qkey;      // [npend][slots]: per-primitive key or NONE
+  double* qval;        // [npend][10][slots]
+};
+
+struct Counters {
+  unsigned long long issued;        // realisations handed out
+  unsigned long long realisations;  // completed and counted
+  unsigned long long failed;
+  unsigned long long segments;
+  unsigned long long spills;
+  unsigned int aborted;
+  unsigned int pad;
+  unsigned int queue_count[2];      // ray queue lengths (ping-pong)
+};
+
+struct Tallies {
+  double* global;   // [6][2]
+  double* rcv;      // [nrcv][2][10][2]
+  double* samp;     // [nsamp][2][2]
+  unsigned long long* samp_count;  // [nsamp]
+  double* sxr;      // [nsamp][nrcv][2][10][2]
+  double* prim;     // [nprim][2][10][2]
+};
+
+struct SolveParams {
+  uint64_t N;
+  uint64_t max_failed;
+  uint64_t key;
+  uint64_t counter;
+  uint32_t slots;
+  uint32_t npend;
+};
+
+// -------------------------------------------------------------------------
+// Wave helpers (64-wide)
+// -------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ void atomic_add(double* p, double v) {
+  if (v != 0) unsafeAtomicAdd(p, v);
+}
+
+// Adds the NV values of each participating lane at base[key*NV + i]. Lanes
+// sharing the first participant's key are reduced across the wave and
+// flushed by one lane; the others use per-lane atomics. Must be called by
+// the whole wave.
+template <int NV>
+__device__ void keyed_add(bool has, uint64_t key, const double* v, double* base) {
+  const uint64_t mask = __ballot(has);
+  if (!mask) return;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  const uint64_t lkey = __shfl(key, leader, 64);
+  const bool match = has && key == lkey;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double s = wave_sum(match ? v[i] : 0.0);
+    if ((int)lane_id() == leader) atomic_add(base + lkey * NV + i, s);
+  }
+  if (has && !match) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) atomic_add(base + key * NV + i, v[i]);
+  }
+}
+
+__device__ __forceinline__ unsigned long long wave_count_add(bool pred, unsigned long long* ctr) {
+  const uint64_t mask = __ballot(pred);
+  if (!mask) return 0;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  unsigned long long base = 0;
+  if ((int)lane_id() == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
+  return base + (unsigned long long)__popcll(below);
+}
+
+// -------------------------------------------------------------------------
+// Traversal with the reference hit filter
+// -------------------------------------------------------------------------
+struct FRay {
+  float ox, oy, oz, ix, iy, iz;
+};
+
+__device__ __forceinline__ FRay make_fray(d3 o, d3 d) {
+  auto inv = [](double x) -> float {
+    const float f = (float)x;
+    return 1.0f / (fabsf(f) > 1e-30f ? f : copysignf(1e-30f, f));
+  };
+  return FRay{(float)o.x, (float)o.y, (float)o.z, inv(d.x), inv(d.y), inv(d.z)};
+}
+
+__device__ __forceinline__ bool box_hit(const FRay& r, const float* lo, const float* hi,
+                                        float tmin, float tmax, float& tnear) {
+  const float t0x = (lo[0] - r.ox) * r.ix, t1x = (hi[0] - r.ox) * r.ix;
+  const float t0y = (lo[1] - r.oy) * r.iy, t1y = (hi[1] - r.oy) * r.iy;
+  const float t0z = (lo[2] - r.oz) * r.iz, t1z = (hi[2] - r.oz) * r.iz;
+  const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+  const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+  tnear = tn;
+  return tn <= tf;
+}
+
+__device__ __forceinline__ double ray_triangle(d3 o, d3 d, d3 v0, d3 v1, d3 v2) {
+  const d3 e1 = v1 - v0, e2 = v2 - v0;
+  const d3 p = cross(d, e2);
+  const double det = dot(e1, p);
+  if (det == 0) return -1;
+  const double inv = 1.0 / det;
+  const d3 s = o - v0;
+  const double u = dot(s, p) * inv;
+  if (u < 0 || u > 1) return -1;
+  const d3 q = cross(s, e1);
+  const double v = dot(d, q) * inv;
+  if (v < 0 || u + v > 1) return -1;
+  return dot(e2, q) * inv;
+}
+
+struct TraceState {
+  double tbest;
+  double qdst;
+  d3 qn;
+  d3 ng;  // world geometry normal of the best hit (unnormalised)
+  uint32_t inst, prim, sshape;
+};
+
+// Returns true if the candidate is accepted (src/ssol_scene.c:438-536).
+__device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t inst_id,
+                       const ssolh_instance& inst, const ssolh_triangle& tri, double t,
+                       TraceState& ts, d3& ng_out) {
+  const uint32_t sshape_from = r.flags & 0xFFFFu;
+  const uint32_t side_from = (r.flags >> RF_SIDE_SHIFT) & 3u;
+  const ssolh_object obj = S.objects[inst.object];
+  const ssolh_sshape ss = S.sshapes[obj.sshape_first + tri.sshape];
+  const ssolh_shape& sh = S.shapes[ss.shape];
+  uint32_t side;
+  double dst = 0;
+  d3 qn = mk(0, 0, 0);
+  if (t < r.tmin) return false;
+  if (sh.type == SSOLH_SHAPE_MESH) {
+    if (t <= 1.e-6 || t <= r.tmin ||
+        (tri.prim == r.prim_from && tri.sshape == sshape_from && inst_id == r.inst_from))
+      return false;
+    const d3 v0 = ld3(tri.v0), v1 = ld3(tri.v1), v2 = ld3(tri.v2);
+    const d3 n_obj = cross(v2 - v0, v1 - v0);
+    ng_out = tmul(inst.inv_linear, n_obj);
+    side = dot(ng_out, d) < 0 ? SSOL_FRONT_ : SSOL_BACK_;
+  } else {
+    const ssolh_inst_sshape& is = S.isshapes[inst.isshape_first + tri.sshape];
+    const d3 lo = mul(is.R_inv, o - ld3(is.T));
+    const d3 ldir = mul(is.R_inv, d);
+    d3 grad;
+    if (!quadric_hit(sh.quadric, sh.qparam, lo, ldir, t, dst, grad)) return false;
+    if (!(dst < (double)FLT_MAX)) return false;
+    if ((double)(float)dst <= r.tmin) return false;
+    qn = normalize(tmul(is.R_inv, grad));
+    side = dot(d, qn) < 0 ? SSOL_FRONT_ : SSOL_BACK_;
+    if (inst_id == r.inst_from && tri.sshape == sshape_from && side != side_from) return false;
+  }
+  if (r.flags & RF_REVERSED) side = side == SSOL_FRONT_ ? SSOL_BACK_ : SSOL_FRONT_;
+  const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? ss.mtl_front : ss.mtl_back];
+  if (m.type == SSOLH_MTL_VIRTUAL) {
+    if (r.flags & RF_DISCARD_VIRTUAL) return false;
+    if (!(inst.receiver_mask & side)) return false;
+  }
+  if (sh.type != SSOLH_SHAPE_MESH && ts.qdst >= dst) {
+    ts.qdst = dst;
+    ts.qn = qn;
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+k_trace(DScene S, const RayRec* __restrict__ rays, HitRec* __restrict__ hits,
+        const unsigned int* __restrict__ count_ptr, unsigned long long* segments) {
+  __shared__ uint32_t lds_stack[STACK * BLOCK];
+  const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
+  const uint32_t count = *count_ptr;
+  const bool live = q < count;
+  // one segment per ray, counted per wave
+  {
+    const uint64_t m = __ballot(live);
+    if (lane_id() == 0 && m) atomicAdd(segments, (unsigned long long)__popcll(m));
+  }
+  if (!live) return;
+
+  const RayRec r = rays[q];
+  const d3 ow = mk(r.ox, r.oy, r.oz), dw = mk(r.dx, r.dy, r.dz);
+  const bool any_hit = (r.flags & RF_ANY_HIT) != 0;
+  TraceState ts;
+  ts.tbest = INFINITY;
+  ts.qdst = INFINITY;
+  ts.qn = mk(0, 0, 0);
+  ts.ng = mk(0, 0, 0);
+  ts.inst = NONE;
+  ts.prim = 0;
+  ts.sshape = 0;
+
+  uint32_t* stack = lds_stack + threadIdx.x;
+  int sp = 0;
+  const FRay fw = make_fray(ow, dw);
+  FRay fr = fw;
+  d3 o = ow, d = dw;        // current-level ray (double)
+  uint32_t cur_inst = NONE;  // instance whose BLAS is being traversed
+  ssolh_instance inst{};
+  const float ftmin = (float)r.tmin;
+  bool done = false;
+
+  uint32_t ref = S.tlas_root;  // inner node reference
+  for (;;) {
+    if (ref == SENTINEL) {
+      // back to the TLAS
+      cur_inst = NONE;
+      fr = fw;
+      o = ow;
+      d = dw;
+    } else if (ref & LEAF) {
+      const uint32_t first = ref & 0x0FFFFFFFu;
+      const uint32_t cnt = ((ref >> 28) & 7u) + 1u;
+      if (cur_inst == NONE) {
+        // TLAS leaf: one instance, enter its BLAS
+        cur_inst = S.tlas_inst[first];
+        inst = S.insts[cur_inst];
+        const d3 T = mk(inst.transform[9], inst.transform[10], inst.transform[11]);
+        o = mul(inst.inv_linear, ow - T);
+        d = mul(inst.inv_linear, dw);
+        fr = make_fray(o, d);
+        if (sp < STACK) { stack[sp * BLOCK] = SENTINEL; ++sp; }
+        ref = S.objects[inst.object].blas_root;
+        continue;
+      }
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const ssolh_triangle tri = S.tris[first + k];
+        const double t = ray_triangle(o, d, ld3(tri.v0), ld3(tri.v1), ld3(tri.v2));
+        if (!(t >= r.tmin && t < ts.tbest)) continue;
+        d3 ng;
+        if (!filter(S, r, ow, dw, cur_inst, inst, tri, t, ts, ng)) continue;
+        ts.tbest = t;
+        ts.inst = cur_inst;
+        ts.prim = tri.prim;
+        ts.sshape = tri.sshape;
+        ts.ng = ng;
+        if (any_hit) { done = true; break; }
+      }
+      if (done) break;
+    } else {
+      const ssolh_bvh_node n = S.nodes[ref];
+      const float tmax = ts.tbest < (double)FLT_MAX ? (float)ts.tbest * 1.0000002f + 1e-30f : FLT_MAX;
+      float tn0, tn1;
+      const bool h0 = n.count[0] != SSOLH_BVH_EMPTY && box_hit(fr, n.lo[0], n.hi[0], ftmin, tmax, tn0);
+      const bool h1 = n.count[1] != SSOLH_BVH_EMPTY && box_hit(fr, n.lo[1], n.hi[1], ftmin, tmax, tn1);
+      auto enc = [&](int c) -> uint32_t {
+        return n.count[c] == 0 ? n.child[c] : (LEAF | ((n.count[c] - 1u) << 28) | n.child[c]);
+      };
+      if (h0 && h1) {
+        const bool near0 = tn0 <= tn1;
+        const uint32_t rn = enc(near0 ? 0 : 1), rf = enc(near0 ? 1 : 0);
+        if (sp < STACK) { stack[sp * BLOCK] = rf; ++sp; }
+        ref = rn;
+        continue;
+      } else if (h0) {
+        ref = enc(0);
+        continue;
+      } else if (h1) {
+        ref = enc(1);
+        continue;
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    ref = stack[sp * BLOCK];
+  }
+
+  HitRec h;
+  h.t = ts.tbest;
+  h.qdst = ts.qdst;
+  h.qnx = ts.qn.x; h.qny = ts.qn.y; h.qnz = ts.qn.z;
+  h.ngx = (float)ts.ng.x; h.ngy = (float)ts.ng.y; h.ngz = (float)ts.ng.z;
+  h.inst = ts.inst;
+  h.prim = ts.prim;
+  h.sshape = ts.sshape;
+  hits[q] = h;
+}
+
+// -------------------------------------------------------------------------
+// Path generation (reference point_init, src/ssol_solver.c:207-358)
+// -------------------------------------------------------------------------
+struct Emit {
+  bool has;
+  RayRec r;
+};
+
+__device__ uint32_t start_medium(const DScene& S, const ssolh_material& m, uint32_t side) {
+  if (m.type == SSOLH_MTL_DIELECTRIC || m.type == SSOLH_MTL_THIN_DIELECTRIC)
+    return (uint32_t)(side == SSOL_FRONT_ ? m.medium_in : m.medium_out);
+  return S.air;
+}
+
+__device__ void generate(const DScene& S, const PathState& P, const SolveParams& prm,
+                         uint32_t slot, uint64_t path, Emit& e, int& samp_slot) {
+  PathRng rng;
+  rng.init(prm.key, prm.counter + path, 0);
+  // area-weighted primitive through the alias table, uniform point
+  const double u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
+  const double x = u0 * (double)S.nstris;
+  uint32_t i = (uint32_t)x;
+  if (i >= S.nstris) i = S.nstris - 1;
+  const ssolh_alias al = S.alias[i];
+  const uint32_t idx = (x - (double)i) < al.prob ? i : al.alias;
+  const ssolh_sample_tri st = S.stris[idx];
+  const ssolh_instance inst = S.insts[st.instance];
+  const double sq = sqrt(u1);
+  const double b0 = 1 - sq, b1 = sq * (1 - u2), b2 = sq * u2;
+  const d3 v0 = ld3(st.v0), v1 = ld3(st.v1), v2 = ld3(st.v2);
+  const d3 T = mk(inst.transform[9], inst.transform[10], inst.transform[11]);
+  d3 pos = mul(inst.transform, v0 * b0 + v1 * b1 + v2 * b2) + T;
+  const d3 ns = normalize(tmul(inst.inv_linear, cross(v2 - v0, v1 - v0)));
+
+  const d3 dir = sample_sun(S.sun, rng);
+  const double wl = sample_wavelength(S.spectra, S.sun.wl_spectrum, rng.next());
+
+  const ssolh_object obj = S.objects[inst.object];
+  const ssolh_sshape ss = S.sshapes[obj.sshape_first + st.sshape];
+  const ssolh_shape& sh = S.shapes[ss.shape];
+  d3 N = ns;
+  if (sh.type == SSOLH_SHAPE_PUNCHED) {
+    // lift onto the quadric (src/ssol_shape.c:1408-1447)
+    const ssolh_inst_sshape& is = S.isshapes[inst.isshape_first + st.sshape];
+    d3 loc = mul(is.R_inv, pos - ld3(is.T));
+    loc.z = quadric_z(sh.quadric, sh.qparam, loc.x, loc.y);
+    N = normalize(tmul(is.R_inv, quadric_grad(sh.quadric, sh.qparam, loc)));
+    pos = mul(is.R, loc) + ld3(is.T);
+  }
+  uint32_t side = SSOL_FRONT_;
+  if (dot(N, dir) >= 0) { side = SSOL_BACK_; N = -N; }
+
+  const d3 sun0 = ld3(S.sun.dir);
+  const double surface_sun_cos = dot(N, dir);
+  const double surface_sun0_cos = fabs(dot(sun0, N));
+  const double sun0_sun_cos = dot(sun0, dir);
+  const double proxy_cos = sh.type == SSOLH_SHAPE_MESH ? 1.0 : fabs(dot(ns, N));
+  const double w0 = S.sun.dni * S.S_proxy * fabs(surface_sun_cos / (proxy_cos * sun0_sun_cos));
+  const double cosf = S.S_proxy / S.S * surface_sun0_cos / proxy_cos;
+
+  const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? ss.mtl_front : ss.mtl_back];
+  const uint32_t med = start_medium(S, m, side);
+
+  P.pos_wl[slot] = make_double4(pos.x, pos.y, pos.z, wl);
+  P.dir_cos[slot] = make_double4(dir.x, dir.y, dir.z, cosf);
+  P.prev_init[slot] = make_double4(w0, w0, w0, w0);
+  P.out_in0[slot] = make_double4(0, 0, 0, 0);
+  P.nrm_t[slot] = make_double4(N.x, N.y, N.z, 0);
+  P.pend[slot] = make_double4(0, 0, 0, 0);
+  P.meta[slot] = make_uint4((uint32_t)path, (uint32_t)(path >> 32), rng.block, PF_ACTIVE);
+  P.cur[slot] = make_uint4(st.instance, st.prim, st.sshape | (side << 16), (uint32_t)inst.sampled);
+  P.misc[slot] = make_uint4(0, med | (med << 16), 0, 64);
+  for (uint32_t k = 0; k < prm.npend; ++k) {
+    P.pkey[(size_t)k * prm.slots + slot] = NONE;
+    P.qkey[(size_t)k * prm.slots + slot] = NONE;
+  }
+  samp_slot = inst.sampled;
+
+  // shadow ray toward the sun (src/ssol_solver.c:325-352)
+  e.has = true;
+  e.r.ox = pos.x; e.r.oy = pos.y; e.r.oz = pos.z; e.r.tmin = 0;
+  e.r.dx = -dir.x; e.r.dy = -dir.y; e.r.dz = -dir.z; e.r.pad = 0;
+  e.r.slot = slot;
+  e.r.inst_from = st.instance;
+  e.r.prim_from = st.prim;
+  e.r.flags = st.sshape | (side << RF_SIDE_SHIFT) | RF_DISCARD_VIRTUAL | RF_REVERSED | RF_ANY_HIT;
+}
+
+// Append the live rays of the wave to the next queue, one atomic per wave
+__device__ void emit_rays(const Emit& e, RayRec* out, unsigned int* count) {
+  const uint64_t mask = __ballot(e.has);
+  if (!mask) return;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  unsigned int base = 0;
+  if ((int)lane_id() == leader) base = atomicAdd(count, (unsigned int)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (e.has) {
+    const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
+    out[base + (unsigned)__popcll(below)] = e.r;
+  }
+}
+
+__device__ void flush_sampled_count(bool has, int samp_slot, const Tallies& T) {
+  const uint64_t key = has ? (uint64_t)samp_slot : 0;
+  const uint64_t mask = __ballot(has);
+  if (!mask) return;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  const uint64_t lkey = __shfl(key, leader, 64);
+  const bool match = has && key == lkey;
+  const uint64_t mm = __ballot(match);
+  if ((int)lane_id() == leader) atomicAdd(T.samp_count + lkey, (unsigned long long)__popcll(mm));
+  if (has && !match) atomicAdd(T.samp_count + key, 1ull);
+}
+
+__global__ void __launch_bounds__(BLOCK)
+k_generate(DScene S, PathState P, SolveParams prm, Tallies T, RayRec* out,
+           unsigned int* out_count, Counters* C) {
+  const uint32_t slot = blockIdx.x * BLOCK + threadIdx.x;
+  Emit e;
+  e.has = false;
+  int samp_slot = 0;
+  bool gen = false;
+  if (slot < prm.slots) {
+    P.meta[slot] = make_uint4(0, 0, 0, 0);
+  }
+  const unsigned long long id = wave_count_add(slot < prm.slots, &C->issued);
+  if (slot < prm.slots && id < prm.N) {
+    generate(S, P, prm, slot, id, e, samp_slot);
+    gen = true;
+  }
+  flush_sampled_count(gen, samp_slot, T);
+  emit_rays(e, out, out_count);
+}
+
+// -------------------------------------------------------------------------
+// Shading (reference trace_radiative_path loop body, src/ssol_solver.c:939-1086)
+// -------------------------------------------------------------------------
+enum { FIN_NONE = 0, FIN_DONE = 1, FIN_KILLED = 2, FIN_FAILED = 3 };
+
+struct Finish {
+  int kind;
+  double g[6];      // cos, absorbed_by_receivers, shadowed, missing, extinct, other
+  double factor;    // survivor factor applied to pending entries
+  int samp_slot;
+};
+
+// Add the 10 receiver weights of a hit to the per-path pending entries;
+// false when every entry holds another key.
+__device__ bool pend_add(uint32_t* keys, double* vals, size_t stride, uint32_t npend,
+                         uint32_t key, const double* w) {
+  int slot_k = -1, free_k = -1;
+  for (uint32_t k = 0; k < npend; ++k) {
+    const uint32_t kk = keys[k * stride];
+    if (kk == key) { slot_k = (int)k; break; }
+    if (kk == NONE && free_k < 0) free_k = (int)k;
+  }
+  if (slot_k < 0 && free_k >= 0) {
+    slot_k = free_k;
+    keys[slot_k * stride] = key;
+    for (int f = 0; f < 10; ++f) vals[((size_t)slot_k * 10 + f) * stride] = 0;
+  }
+  if (slot_k < 0) return false;
+  for (int f = 0; f < 10; ++f) vals[((size_t)slot_k * 10 + f) * stride] += w[f];
+  return true;
+}
+
+// Weights that found no pending entry are flushed as a term of their own
+__device__ void spill(double* base, uint64_t key, const double* w, unsigned long long* spills) {
+  atomicAdd(spills, 1ull);
+  for (int f = 0; f < 10; ++f) {
+    atomic_add(base + key * 20 + 2 * f, w[f]);
+    atomic_add(base + key * 20 + 2 * f + 1, w[f] * w[f]);
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK)
+k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restrict__ rays,
+        const HitRec* __restrict__ hits, const unsigned int* __restrict__ count_ptr,
+        RayRec* __restrict__ out, unsigned int* out_count, Counters* C) {
+  const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
+  const uint32_t count = *count_ptr;
+  const bool live = q < count;
+  const size_t stride = prm.slots;
+
+  Emit e;
+  e.has = false;
+  Finish fin;
+  fin.kind = FIN_NONE;
+  fin.factor = 1;
+  fin.samp_slot = 0;
+  for (int i = 0; i < 6; ++i) fin.g[i] = 0;
+  uint32_t slot = 0;
+
+  if (live) {
+    const RayRec r = rays[q];
+    const HitRec h = hits[q];
+    slot = r.slot;
+    uint4 meta = P.meta[slot];
+    uint4 cur = P.cur[slot];
+    uint4 misc = P.misc[slot];
+    double4 pw = P.pos_wl[slot];
+    double4 dc = P.dir_cos[slot];
+    double4 pv = P.prev_init[slot];
+    double4 oi = P.out_in0[slot];
+    double4 nt = P.nrm_t[slot];
+    double4 pd = P.pend[slot];
+    const uint64_t path = (uint64_t)meta.x | ((uint64_t)meta.y << 32);
+    PathRng rng;
+    rng.init(prm.key, prm.counter + path, meta.z);
+    uint32_t flags = meta.w;
+    fin.samp_slot = (int)cur.w;
+
+    uint32_t in_med = misc.y & 0xFFFFu, out_med = misc.y >> 16;
+    uint32_t depth = misc.x, score = misc.z, rinterval = misc.w;
+    d3 pos = mk(pw.x, pw.y, pw.z);
+    const double wl = pw.w;
+    d3 dir = mk(dc.x, dc.y, dc.z);
+    d3 N = mk(nt.x, nt.y, nt.z);
+    double hit_t = 0;
+    d3 rorg = pos, rdir = dir;  // origin/direction of the ray of the point
+    bool head = false;
+    uint32_t inst_id = cur.x, prim = cur.y, sshape = cur.z & 0xFFFFu, side = cur.z >> 16;
+
+    if (!(flags & PF_STAGE_BOUNCE)) {
+      // result of the shadow ray
+      if (h.inst != NONE) {
+        fin.kind = FIN_DONE;
+        fin.g[0] = dc.w;      // cos factor
+        fin.g[2] = pv.w;      // shadowed = initial flux
+      } else {
+        head = true;
+        hit_t = 0;
+      }
+    } else {
+      // tail of the iteration that traced this ray
+      const bool found = h.inst != NONE;
+      const bool hit_virtual = (flags & PF_HIT_VIRTUAL) != 0;
+      const bool in_atm = (flags & PF_IN_ATM) != 0;
+      bool last = false;
+      if (!found) {
+        last = true;
+        if (out_med != S.air) fin.kind = FIN_FAILED;  // ray lost outside air
+      }
+      if (fin.kind == FIN_NONE && (last || !hit_virtual)) {
+        const double absorbed = pv.x - oi.w;
+        if (in_atm) pd.y += absorbed; else pd.z += absorbed;
+        if (last) {
+          fin.kind = FIN_DONE;
+          fin.g[3] = oi.x;  // missing
+        } else {
+          pv.x = oi.x; pv.y = oi.y; pv.z = oi.z;
+        }
+      }
+      if (fin.kind == FIN_NONE) {
+        depth += hit_virtual ? 0u : 1u;
+        if (depth > rinterval && depth % rinterval == 1) {
+          if (rng.next() > 0.5) {
+            score += 1;
+            rinterval = 16;
+          } else {
+            fin.kind = FIN_KILLED;
+          }
+        }
+      }
+      if (fin.kind == FIN_NONE) {
+        // point_update_from_hit (src/ssol_solver.c:360-411)
+        rorg = mk(r.ox, r.oy, r.oz);
+        rdir = mk(r.dx, r.dy, r.dz);
+        inst_id = h.inst;
+        prim = h.prim;
+        sshape = h.sshape;
+        const ssolh_instance inst = S.insts[inst_id];
+        const ssolh_sshape ss = S.sshapes[S.objects[inst.object].sshape_first + sshape];
+        if (S.shapes[ss.shape].type == SSOLH_SHAPE_MESH) {
+          N = normalize(mk(h.ngx, h.ngy, h.ngz));
+          pos = rorg + rdir * h.t;
+        } else {
+          N = normalize(mk(h.qnx, h.qny, h.qnz));
+          pos = rorg + dir * h.qdst;
+        }
+        if (dot(dir, N) < 0) side = SSOL_FRONT_;
+        else { side = SSOL_BACK_; N = -N; }
+        in_med = out_med;
+        hit_t = h.t;
+        head = true;
+      }
+    }
+
+    if (head) {
+      if (!(flags & PF_STAGE_BOUNCE)) { rorg = pos; rdir = dir; }
+      const ssolh_instance inst = S.insts[inst_id];
+      const ssolh_sshape ss = S.sshapes[S.objects[inst.object].sshape_first + sshape];
+      const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? ss.mtl_front : ss.mtl_back];
+      const bool in_atm = in_med == S.air;
+      const bool receiver = (inst.receiver_mask & side) != 0;
+      const bool virt = m.type == SSOLH_MTL_VIRTUAL;
+      double trans = 1;
+      if (hit_t > 0) {
+        const double k = data_value(S.spectra, S.media[in_med].extinction, wl);
+        if (k > 0) trans = exp(-k * hit_t);
+      }
+      const double in0 = pv.x * trans;
+      const double in1 = in_atm ? pv.y : pv.y * trans;
+      const double in2 = !in_atm ? pv.z : pv.z * trans;
+      double out0, out1, out2, kabs;
+      uint32_t next_med = in_med;
+      if (virt) {
+        out0 = in0; out1 = in1; out2 = in2; kabs = 0;
+      } else {
+        double eta_out = 1, eta_in = 1, slab_eta = 1, slab_abs = 0;
+        if (m.type == SSOLH_MTL_DIELECTRIC) {
+          if ((uint32_t)m.medium_out != in_med) fin.kind = FIN_FAILED;
+          eta_out = data_value(S.spectra, S.media[m.medium_out].refractive_index, wl);
+          eta_in = data_value(S.spectra, S.media[m.medium_in].refractive_index, wl);
+        } else if (m.type == SSOLH_MTL_THIN_DIELECTRIC) {
+          eta_out = data_value(S.spectra, S.media[m.medium_out].refractive_index, wl);
+          slab_eta = data_value(S.spectra, m.slab_eta, wl);
+          slab_abs = data_value(S.spectra, m.slab_absorption, wl);
+        }
+        double w = 0;
+        int lobe = LOBE_NONE;
+        const d3 wi = -dir;
+        if (fin.kind == FIN_NONE && dot(wi, N) > 0) {
+          d3 wo;
+          w = bsdf_sample(m, eta_out, eta_in, slab_eta, slab_abs, wi, N, rng, wo, lobe);
+          const double c = dot(N, wo);
+          if ((c > 0 && lobe == LOBE_TRANSMIT) || (c < 0 && lobe == LOBE_REFLECT)) w = 0;
+          dir = wo;
+        }
+        kabs = 1 - w;
+        out0 = in0 * w;
+        out1 = in1 * w;
+        out2 = receiver ? in2 * w : in2;
+        if (lobe == LOBE_TRANSMIT && m.type == SSOLH_MTL_DIELECTRIC)
+          next_med = (uint32_t)(m.medium_out == (int32_t)in_med ? m.medium_in : m.medium_out);
+      }
+      if (fin.kind == FIN_NONE) {
+        if (receiver) {
+          pd.x += in0 - out0;
+          double w10[10];
+          w10[0] = in0; w10[1] = in1; w10[2] = in2;
+          w10[3] = in2 - in0; w10[4] = in1 - in0;
+          w10[5] = in0 * kabs; w10[6] = in1 * kabs; w10[7] = in2 * kabs;
+          w10[8] = (in2 - in0) * kabs; w10[9] = (in1 - in0) * kabs;
+          const uint32_t rkey = (uint32_t)inst.receiver * 2u + (side == SSOL_FRONT_ ? 0u : 1u);
+          if (!pend_add(P.pkey + slot, P.pval + slot, stride, prm.npend, rkey, w10)) {
+            spill(T.rcv, rkey, w10, &C->spills);
+            spill(T.sxr, (uint64_t)cur.w * S.nrcv * 2u + rkey, w10, &C->spills);
+          }
+          const ssolh_inst_sshape& is = S.isshapes[inst.isshape_first + sshape];
+          if (is.prim_tally >= 0) {
+            const uint32_t pkey = (uint32_t)(((uint64_t)is.prim_tally + prim) * 2u +
+                                             (side == SSOL_FRONT_ ? 0u : 1u));
+            if (!pend_add(P.qkey + slot, P.qval + slot, stride, prm.npend, pkey, w10))
+              spill(T.prim, pkey, w10, &C->spills);
+          }
+        } else {
+          pd.z += in0 * kabs;
+        }
+        if (out0 == 0) {
+          // weight is zero: the segment ends here
+          const double absorbed = pv.x - in0;
+          if (in_atm) pd.y += absorbed; else pd.z += absorbed;
+          fin.kind = FIN_DONE;
+          fin.g[3] = 0;
+        } else {
+          e.has = true;
+          if (virt) {
+            e.r.ox = rorg.x; e.r.oy = rorg.y; e.r.oz = rorg.z;
+            e.r.dx = rdir.x; e.r.dy = rdir.y; e.r.dz = rdir.z;
+            e.r.tmin = (double)nextafterf((float)hit_t, FLT_MAX);
+          } else {
+            e.r.ox = pos.x; e.r.oy = pos.y; e.r.oz = pos.z;
+            e.r.dx = dir.x; e.r.dy = dir.y; e.r.dz = dir.z;
+            e.r.tmin = 0;
+          }
+          e.r.pad = 0;
+          e.r.slot = slot;
+          e.r.inst_from = inst_id;
+          e.r.prim_from = prim;
+          e.r.flags = sshape | (side << RF_SIDE_SHIFT);
+          flags = PF_ACTIVE | PF_STAGE_BOUNCE | (virt ? PF_HIT_VIRTUAL : 0u) | (in_atm ? PF_IN_ATM : 0u);
+          oi = make_double4(out0, out1, out2, in0);
+          out_med = next_med;
+        }
+      }
+    }
+
+    if (fin.kind == FIN_DONE) {
+      fin.g[0] = dc.w;
+      fin.g[1] = pd.x;
+      fin.g[4] = pd.y;
+      fin.g[5] = pd.z;
+      fin.factor = (double)(1u << score);
+      for (int i = 0; i < 6; ++i) fin.g[i] *= fin.factor;
+    }
+    if (fin.kind == FIN_NONE) {
+      P.pos_wl[slot] = make_double4(pos.x, pos.y, pos.z, wl);
+      P.dir_cos[slot] = make_double4(dir.x, dir.y, dir.z, dc.w);
+      P.prev_init[slot] = pv;
+      P.out_in0[slot] = oi;
+      P.nrm_t[slot] = make_double4(N.x, N.y, N.z, hit_t);
+      P.pend[slot] = pd;
+      P.meta[slot] = make_uint4(meta.x, meta.y, rng.block, flags);
+      P.cur[slot] = make_uint4(inst_id, prim, sshape | (side << 16), cur.w);
+      P.misc[slot] = make_uint4(depth, in_med | (out_med << 16), score, rinterval);
+    }
+  }
+
+  // --- convergent flush of finished realisations -------------------------
+  const bool done = fin.kind == FIN_DONE;
+  {
+    double g2[12];
+    for (int i = 0; i < 6; ++i) {
+      g2[2 * i] = done ? fin.g[i] : 0.0;
+      g2[2 * i + 1] = done ? fin.g[i] * fin.g[i] : 0.0;
+    }
+    keyed_add<12>(done, 0, g2, T.global);
+    double s4[4] = {fin.g[0], fin.g[0] * fin.g[0], fin.g[2], fin.g[2] * fin.g[2]};
+    keyed_add<4>(done, (uint64_t)fin.samp_slot, s4, T.samp);
+    wave_count_add(done || fin.kind == FIN_KILLED, &C->realisations);
+    const unsigned long long nf = wave_count_add(fin.kind == FIN_FAILED, &C->failed);
+    if (fin.kind == FIN_FAILED && nf + 1 >= prm.max_failed) atomicExch(&C->aborted, 1u);
+  }
+  for (uint32_t k = 0; k < prm.npend; ++k) {
+    uint32_t key = NONE;
+    double v[20];
+    if (live && fin.kind != FIN_NONE) {
+      key = P.pkey[k * stride + slot];
+      if (key != NONE) {
+        for (int f = 0; f < 10; ++f) {
+          const double x = P.pval[((size_t)k * 10 + f) * stride + slot] * fin.factor;
+          v[2 * f] = x;
+          v[2 * f + 1] = x * x;
+        }
+      }
+      P.pkey[k * stride + slot] = NONE;
+    }
+    const bool has = done && key != NONE;
+    if (!has) for (int f = 0; f < 20; ++f) v[f] = 0;
+    keyed_add<20>(has, has ? key : 0, v, T.rcv);
+    const uint64_t xkey = has ? (uint64_t)fin.samp_slot * (uint64_t)S.nrcv * 2u + key : 0;
+    keyed_add<20>(has, xkey, v, T.sxr);
+  }
+  for (uint32_t k = 0; k < prm.npend; ++k) {
+    uint32_t key = NONE;
+    double v[20];
+    if (live && fin.kind != FIN_NONE) {
+      key = P.qkey[k * stride + slot];
+      if (key != NONE) {
+        for (int f = 0; f < 10; ++f) {
+          const double x = P.qval[((size_t)k * 10 + f) * stride + slot] * fin.factor;
+          v[2 * f] = x;
+          v[2 * f + 1] = x * x;
+        }
+      }
+      P.qkey[k * stride + slot] = NONE;
+    }
+    const bool has = done && key != NONE;
+    if (has) {
+      for (int f = 0; f < 20; ++f) atomic_add(T.prim + (size_t)key * 20 + f, v[f]);
+    }
+  }
+
+  // --- refill finished slots with new realisations ------------------------
+  const bool want = live && fin.kind != FIN_NONE && !C->aborted;
+  const unsigned long long id = wave_count_add(want, &C->issued);
+  int samp_slot = 0;
+  bool gen = false;
+  if (want && id < prm.N) {
+    generate(S, P, prm, slot, id, e, samp_slot);
+    gen = true;
+  } else if (live && fin.kind != FIN_NONE) {
+    P.meta[slot] = make_uint4(0, 0, 0, 0);
+  }
+  flush_sampled_count(gen, samp_slot, T);
+  emit_rays(e, out, out_count);
+}
+
+// -------------------------------------------------------------------------
+// Deterministic test kernels
+// -------------------------------------------------------------------------
+__global__ void k_test_quadric(int quadric, double4 qp, uint32_t n, const double* org,
+                               const double* dir, const double* hint, double* t, double* nrm) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double p[4] = {qp.x, qp.y, qp.z, qp.w};
+  double tt;
+  d3 g;
+  if (quadric_hit(quadric, p, ld3(org + 3 * i), ld3(dir + 3 * i), hint[i], tt, g)) {
+    t[i] = tt;
+    nrm[3 * i] = g.x; nrm[3 * i + 1] = g.y; nrm[3 * i + 2] = g.z;
+  } else {
+    t[i] = INFINITY;
+    nrm[3 * i] = nrm[3 * i + 1] = nrm[3 * i + 2] = 0;
+  }
+}
+
+struct ArrayRng {
+  const double* u;
+  int n, i;
+};
+
+__global__ void k_test_sun(ssolh_sun sun, uint32_t n, const double* u, double* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // pillbox / gaussian consume exactly two uniforms: run them through the
+  // same formulas as sample_sun with the given values
+  const double u0 = u[2 * i], u1 = u[2 * i + 1];
+  double st, ct, phi;
+  d3 loc, d;
+  if (sun.type == SSOLH_SUN_PILLBOX) {
+    const double s2 = u0 * sun.param[0];
+    st = sqrt(s2); ct = sqrt(1 - s2); phi = u1 * 2 * PI;
+    loc = mk(cos(phi) * st, sin(phi) * st, ct);
+    d = normalize(mul(sun.basis, loc));
+  } else if (sun.type == SSOLH_SUN_GAUSSIAN) {
+    st = sun.param[0] * sqrt(-2 * log(u0));
+    st = st > 1 ? 1 : st;
+    ct = sqrt(1 - st * st); phi = u1 * 2 * PI;
+    loc = mk(cos(phi) * st, sin(phi) * st, ct);
+    d = normalize(mul(sun.basis, loc));
+  } else {
+    d = ld3(sun.dir);
+  }
+  out[3 * i] = d.x; out[3 * i + 1] = d.y; out[3 * i + 2] = d.z;
+}
+
+__global__ void k_test_threefry(uint32_t n, const uint64_t* ctr, const uint64_t* key, uint64_t* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  threefry4x64_20(ctr + 4 * i, key + 4 * i, out + 4 * i);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+};
+
+struct ssolh_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {};
+  // scene
+  std::vector<DevBuf> scene_bufs;
+  DScene scn{};
+  bool has_scene = false;
+  uint32_t nrcv = 0, nsamp = 0;
+  uint64_t nprim = 0;
+  // solve buffers
+  DevBuf state, rays[2], hits, counters, tallies;
+  PathState P{};
+  Tallies T{};
+  size_t state_slots = 0, state_npend = 0;
+  size_t tally_bytes = 0;
+  // last solve
+  Counters last{};
+  double kernel_ms = 0, trace_ms = 0, shade_ms = 0;
+  uint32_t iterations = 0;
+};
+
+namespace {
+
+int dev_alloc(DevBuf& b, size_t bytes) {
+  if (b.n >= bytes && b.p) return 0;
+  if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.n = 0; }
+  if (!bytes) bytes = 16;
+  HIP_TRY(hipMalloc(&b.p, bytes));
+  b.n = bytes;
+  return 0;
+}
+
+template <typename T>
+int upload(ssolh_context* c, const T* src, size_t count, const T** dst) {
+  DevBuf b;
+  const size_t bytes = count * sizeof(T);
+  int r = dev_alloc(b, bytes ? bytes : 16);
+  if (r) return r;
+  if (bytes) HIP_TRY(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+  c->scene_bufs.push_back(b);
+  *dst = (const T*)b.p;
+  return 0;
+}
+
+void free_scene(ssolh_context* c) {
+  for (auto& b : c->scene_bufs) if (b.p) (void)hipFree(b.p);
+  c->scene_bufs.clear();
+  c->has_scene = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+SSOLH_API const char* ssolh_last_error(void) { return g_err; }
+
+SSOLH_API int ssolh_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+SSOLH_API int ssolh_create(int device, ssolh_context** out) {
+  if (!out) { set_err("ssolh_create: NULL output"); return SSOLH_ERR_ARG; }
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device) {
+    set_err("ssolh_create: no HIP device %d (found %d)", device, n);
+    return SSOLH_ERR_NO_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_err("ssolh_create: device %d is %s, the kernels are built for gfx950", device,
+            prop.gcnArchName);
+    return SSOLH_ERR_NO_DEVICE;
+  }
+  ssolh_context* c = new ssolh_context();
+  c->device = device;
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
+  *out = c;
+  return 0;
+}
+
+SSOLH_API int ssolh_destroy(ssolh_context* c) {
+  if (!c) return SSOLH_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_scene(c);
+  for (DevBuf* b : {&c->state, &c->rays[0], &c->rays[1], &c->hits, &c->counters, &c->tallies})
+    if (b->p) (void)hipFree(b->p);
+  for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) {
+  if (!c || !s) { set_err("ssolh_scene_upload: NULL argument"); return SSOLH_ERR_ARG; }
+  if (!s->nsample_tris || !s->ninstances || !s->nnodes) {
+    set_err("ssolh_scene_upload: empty scene (sampled triangles %u, instances %u)",
+            s->nsample_tris, s->ninstances);
+    return SSOLH_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  free_scene(c);
+  DScene& d = c->scn;
+  int r = 0;
+  const double* cdf = nullptr;
+  r |= upload(c, s->spectra, s->nspectra, &d.spectra.sp);
+  r |= upload(c, s->spectra_wl, s->spectra_len, &d.spectra.wl);
+  r |= upload(c, s->spectra_val, s->spectra_len, &d.spectra.val);
+  r |= upload(c, s->spectra_cdf, s->spectra_len, &cdf);
+  d.spectra.cdf = cdf;
+  r |= upload(c, s->media, s->nmedia, &d.media);
+  r |= upload(c, s->materials, s->nmaterials, &d.mtls);
+  r |= upload(c, s->shapes, s->nshapes, &d.shapes);
+  r |= upload(c, s->sshapes, s->nsshapes, &d.sshapes);
+  r |= upload(c, s->objects, s->nobjects, &d.objects);
+  r |= upload(c, s->nodes, s->nnodes, &d.nodes);
+  r |= upload(c, s->triangles, s->ntriangles, &d.tris);
+  r |= upload(c, s->instances, s->ninstances, &d.insts);
+  r |= upload(c, s->tlas_instances, s->ninstances, &d.tlas_inst);
+  r |= upload(c, s->inst_sshapes, s->ninst_sshapes, &d.isshapes);
+  r |= upload(c, s->sample_tris, s->nsample_tris, &d.stris);
+  r |= upload(c, s->sample_alias, s->nsample_tris, &d.alias);
+  if (r) return r;
+  d.air = s->air_medium;
+  d.tlas_root = s->tlas_root;
+  d.nstris = s->nsample_tris;
+  d.sun = s->sun;
+  d.S = s->sampled_area;
+  d.S_proxy = s->sampled_area_proxy;
+  d.nrcv = s->nreceivers;
+  d.nsamp = s->nsampled;
+  c->nrcv = s->nreceivers;
+  c->nsamp = s->nsampled;
+  c->nprim = s->nprim_tallies;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->has_scene = true;
+  return 0;
+}
+
+SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
+  if (!c || !a || !a->realisations) { set_err("ssolh_solve: invalid arguments"); return SSOLH_ERR_ARG; }
+  if (!c->has_scene) { set_err("ssolh_solve: no scene uploaded"); return SSOLH_ERR_ARG; }
+  if (a->precision != SSOLH_FP64) {
+    set_err("ssolh_solve: precision %u not available", a->precision);
+    return SSOLH_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  const uint32_t npend = a->pending_entries ? a->pending_entries : 4;
+  if (npend > MAX_PEND) { set_err("ssolh_solve: at most %d pending entries", MAX_PEND); return SSOLH_ERR_LIMIT; }
+  uint64_t slots64 = a->path_slots ? a->path_slots : (1u << 21);
+  if (slots64 > a->realisations) slots64 = a->realisations;
+  slots64 = (slots64 + BLOCK - 1) / BLOCK * BLOCK;
+  const uint32_t slots = (uint32_t)slots64;
+
+  // path state
+  const size_t n = slots;
+  const size_t bytes_state = n * (6 * sizeof(double4) + 3 * sizeof(uint4)) +
+                             (size_t)npend * n * (2 * sizeof(uint32_t) + 20 * sizeof(double));
+  int r = dev_alloc(c->state, bytes_state);
+  if (r) return r;
+  char* p = (char*)c->state.p;
+  auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) / 256 * 256; return q; };
+  // (256-byte rounding fits inside the allocation: counted below)
+  c->P.pos_wl = (double4*)take(n * sizeof(double4));
+  c->P.dir_cos = (double4*)take(n * sizeof(double4));
+  c->P.prev_init = (double4*)take(n * sizeof(double4));
+  c->P.out_in0 = (double4*)take(n * sizeof(double4));
+  c->P.nrm_t = (double4*)take(n * sizeof(double4));
+  c->P.pend = (double4*)take(n * sizeof(double4));
+  c->P.meta = (uint4*)take(n * sizeof(uint4));
+  c->P.cur = (uint4*)take(n * sizeof(uint4));
+  c->P.misc = (uint4*)take(n * sizeof(uint4));
+  c->P.pkey = (uint32_t*)take(npend * n * sizeof(uint32_t));
+  c->P.qkey = (uint32_t*)take(npend * n * sizeof(uint32_t));
+  c->P.pval = (double*)take(npend * n * 10 * sizeof(double));
+  c->P.qval = (double*)take(npend * n * 10 * sizeof(double));
+  if ((size_t)(p - (char*)c->state.p) > c->state.n) {
+    // the rounding overflowed the first estimate: reallocate with slack
+    r = dev_alloc(c->state, bytes_state + 16 * 256);
+    if (r) return r;
+    p = (char*)c->state.p;
+    c->P.pos_wl = (double4*)take(n * sizeof(double4));
+    c->P.dir_cos = (double4*)take(n * sizeof(double4));
+    c->P.prev_init = (double4*)take(n * sizeof(double4));
+    c->P.out_in0 = (double4*)take(n * sizeof(double4));
+    c->P.nrm_t = (double4*)take(n * sizeof(double4));
+    c->P.pend = (double4*)take(n * sizeof(double4));
+    c->P.meta = (uint4*)take(n * sizeof(uint4));
+    c->P.cur = (uint4*)take(n * sizeof(uint4));
+    c->P.misc = (uint4*)take(n * sizeof(uint4));
+    c->P.pkey = (uint32_t*)take(npend * n * sizeof(uint32_t));
+    c->P.qkey = (uint32_t*)take(npend * n * sizeof(uint32_t));
+    c->P.pval = (double*)take(npend * n * 10 * sizeof(double));
+    c->P.qval = (double*)take(npend * n * 10 * sizeof(double));
+  }
+  for (int i = 0; i < 2; ++i) if ((r = dev_alloc(c->rays[i], n * sizeof(RayRec)))) return r;
+  if ((r = dev_alloc(c->hits, n * sizeof(HitRec)))) return r;
+  if ((r = dev_alloc(c->counters, sizeof(Counters)))) return r;
+
+  // tallies
+  const size_t n_global = 12, n_rcv = (size_t)c->nrcv * 40, n_samp = (size_t)c->nsamp * 4;
+  const size_t n_sxr = (size_t)c->nsamp * c->nrcv * 40, n_prim = (size_t)c->nprim * 40;
+  const size_t tally_doubles = n_global + n_rcv + n_samp + n_sxr + n_prim;
+  const size_t tally_bytes = tally_doubles * sizeof(double) + (size_t)c->nsamp * 8 + 64;
+  if ((r = dev_alloc(c->tallies, tally_bytes))) return r;
+  c->tally_bytes = tally_bytes;
+  double* tb = (double*)c->tallies.p;
+  c->T.global = tb;
+  c->T.rcv = tb + n_global;
+  c->T.samp = c->T.rcv + n_rcv;
+  c->T.sxr = c->T.samp + n_samp;
+  c->T.prim = c->T.sxr + n_sxr;
+  c->T.samp_count = (unsigned long long*)(c->T.prim + n_prim);
+  HIP_TRY(hipMemsetAsync(c->tallies.p, 0, tally_bytes, c->stream));
+  HIP_TRY(hipMemsetAsync(c->counters.p, 0, sizeof(Counters), c->stream));
+
+  SolveParams prm;
+  prm.N = a->realisations;
+  prm.max_failed = a->max_failed;
+  prm.key = a->rng_key;
+  prm.counter = a->rng_counter;
+  prm.slots = slots;
+  prm.npend = npend;
+
+  Counters* ctr = (Counters*)c->counters.p;
+  RayRec* rq[2] = {(RayRec*)c->rays[0].p, (RayRec*)c->rays[1].p};
+  HitRec* hq = (HitRec*)c->hits.p;
+  unsigned int* qcount[2] = {&ctr->queue_count[0], &ctr->queue_count[1]};
+
+  float ms_trace = 0, ms_shade = 0;
+  HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  const uint32_t gblocks = slots / BLOCK;
+  k_generate<<<gblocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, rq[0], qcount[0], ctr);
+  HIP_TRY(hipGetLastError());
+  int cur = 0;
+  uint32_t iter = 0;
+  unsigned int count = 0;
+  HIP_TRY(hipMemcpyAsync(&count, qcount[0], sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  while (count) {
+    if (iter >= 1000000u) {
+      set_err("ssolh_solve: the wavefront did not drain after %u iterations", iter);
+      return SSOLH_ERR_LIMIT;
+    }
+    const int nxt = cur ^ 1;
+    HIP_TRY(hipMemsetAsync(qcount[nxt], 0, sizeof(unsigned int), c->stream));
+    const uint32_t blocks = (count + BLOCK - 1) / BLOCK;
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    k_trace<<<blocks, BLOCK, 0, c->stream>>>(c->scn, rq[cur], hq, qcount[cur], &ctr->segments);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    k_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, rq[cur], hq, qcount[cur],
+                                             rq[nxt], qcount[nxt], ctr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+    HIP_TRY(hipMemcpyAsync(&count, qcount[nxt], sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float t1 = 0, t2 = 0;
+    (void)hipEventElapsedTime(&t1, c->ev[1], c->ev[2]);
+    (void)hipEventElapsedTime(&t2, c->ev[2], c->ev[3]);
+    ms_trace += t1;
+    ms_shade += t2;
+    cur = nxt;
+    ++iter;
+  }
+  float total = 0;
+  HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipEventElapsedTime(&total, c->ev[0], c->ev[3]);
+  HIP_TRY(hipMemcpy(&c->last, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
+  c->kernel_ms = total;
+  c->trace_ms = ms_trace;
+  c->shade_ms = ms_shade;
+  c->iterations = iter;
+  return 0;
+}
+
+SSOLH_API int ssolh_tallies_download(ssolh_context* c, struct ssolh_tallies* o) {
+  if (!c || !o) { set_err("ssolh_tallies_download: NULL argument"); return SSOLH_ERR_ARG; }
+  HIP_TRY(hipSetDevice(c->device));
+  const size_t n_rcv = (size_t)c->nrcv * 40, n_samp = (size_t)c->nsamp * 4;
+  const size_t n_sxr = (size_t)c->nsamp * c->nrcv * 40, n_prim = (size_t)c->nprim * 40;
+  HIP_TRY(hipMemcpy(o->global, c->T.global, 12 * sizeof(double), hipMemcpyDeviceToHost));
+  if (o->receivers && n_rcv) HIP_TRY(hipMemcpy(o->receivers, c->T.rcv, n_rcv * 8, hipMemcpyDeviceToHost));
+  if (o->sampled && n_samp) HIP_TRY(hipMemcpy(o->sampled, c->T.samp, n_samp * 8, hipMemcpyDeviceToHost));
+  if (o->sampled_count && c->nsamp)
+    HIP_TRY(hipMemcpy(o->sampled_count, c->T.samp_count, c->nsamp * 8, hipMemcpyDeviceToHost));
+  if (o->sampled_x_receivers && n_sxr)
+    HIP_TRY(hipMemcpy(o->sampled_x_receivers, c->T.sxr, n_sxr * 8, hipMemcpyDeviceToHost));
+  if (o->primitives && n_prim) HIP_TRY(hipMemcpy(o->primitives, c->T.prim, n_prim * 8, hipMemcpyDeviceToHost));
+  o->realisation_count = c->last.realisations;
+  o->failed_count = c->last.failed;
+  o->segments = c->last.segments;
+  o->pending_spills = c->last.spills;
+  o->aborted = c->last.aborted;
+  o->iterations = c->iterations;
+  o->kernel_ms = c->kernel_ms;
+  o->trace_ms = c->trace_ms;
+  o->shade_ms = c->shade_ms;
+  return 0;
+}
+
+SSOLH_API int ssolh_test_quadric_hits(ssolh_context* c, int quadric, const double qparam[4],
+                                      uint32_t n, const double* org, const double* dir,
+                                      const double* hint, double* t, double* normal) {
+  if (!c || !n) return SSOLH_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->device));
+  double *d_o, *d_d, *d_h, *d_t, *d_n;
+  HIP_TRY(hipMalloc(&d_o, n * 24)); HIP_TRY(hipMalloc(&d_d, n * 24));
+  HIP_TRY(hipMalloc(&d_h, n * 8)); HIP_TRY(hipMalloc(&d_t, n * 8)); HIP_TRY(hipMalloc(&d_n, n * 24));
+  HIP_TRY(hipMemcpy(d_o, org, n * 24, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_d, dir, n * 24, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_h, hint, n * 8, hipMemcpyHostToDevice));
+  k_test_quadric<<<(n + 127) / 128, 128, 0, c->stream>>>(
+      quadric, make_double4(qparam[0], qparam[1], qparam[2], qparam[3]), n, d_o, d_d, d_h, d_t, d_n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(t, d_t, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(normal, d_n, n * 24, hipMemcpyDeviceToHost));
+  (void)hipFree(d_o); (void)hipFree(d_d); (void)hipFree(d_h); (void)hipFree(d_t); (void)hipFree(d_n);
+  return 0;
+}
+
+SSOLH_API int ssolh_test_sun_dirs(ssolh_context* c, const struct ssolh_sun* sun, uint32_t n,
+                                  const double* u, double* dir) {
+  if (!c || !sun || !n) return SSOLH_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->device));
+  double *d_u, *d_o;
+  HIP_TRY(hipMalloc(&d_u, n * 16)); HIP_TRY(hipMalloc(&d_o, n * 24));
+  HIP_TRY(hipMemcpy(d_u, u, n * 16, hipMemcpyHostToDevice));
+  k_test_sun<<<(n + 127) / 128, 128, 0, c->stream>>>(*sun, n, d_u, d_o);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(dir, d_o, n * 24, hipMemcpyDeviceToHost));
+  (void)hipFree(d_u); (void)hipFree(d_o);
+  return 0;
+}
+
+SSOLH_API int ssolh_test_threefry(ssolh_context* c, uint32_t n, const uint64_t* ctr,
+                                  const uint64_t* key, uint64_t* out) {
+  if (!c || !n) return SSOLH_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->device));
+  uint64_t *d_c, *d_k, *d_o;
+  HIP_TRY(hipMalloc(&d_c, n * 32)); HIP_TRY(hipMalloc(&d_k, n * 32)); HIP_TRY(hipMalloc(&d_o, n * 32));
+  HIP_TRY(hipMemcpy(d_c, ctr, n * 32, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_k, key, n * 32, hipMemcpyHostToDevice));
+  k_test_threefry<<<(n + 127) / 128, 128, 0, c->stream>>>(n, d_c, d_k, d_o);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(out, d_o, n * 32, hipMemcpyDeviceToHost));
+  (void)hipFree(d_c); (void)hipFree(d_k); (void)hipFree(d_o);
+  return 0;
+}
+
+}  // extern "C"

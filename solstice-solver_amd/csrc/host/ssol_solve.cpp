@@ -1,0 +1,891 @@
+// ssol_solve and the estimator getters.
+//
+// ssol_solve (reference src/ssol_solver.c:613-842) checks its arguments and
+// the scene (src/ssol_scene.c:538-558), flattens the scene into the plain
+// arrays of ssolh.h (shader baking, media deduplication, per-object BLAS,
+// instance TLAS, area-weighted alias table over the sampling view, tally
+// layout), and runs the realisations on the GPU through ssolh_*. There is no
+// CPU path. The estimator getters restate src/ssol_estimator.c:112-268 and
+// src/ssol_mc_receiver.c:32-173 over the downloaded tally arrays.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_map>
+
+#include "ssol_internal.hpp"
+
+using namespace ssol;
+
+namespace {
+
+constexpr int GPU_STACK = 48;  // traversal stack entries of the trace kernel
+
+struct DataKey {
+  int spectrum;
+  double real;
+  uint64_t c0, c1;
+};
+
+// Flat scene under construction; owns every array handed to ssolh.
+struct Flat {
+  std::vector<ssolh_spectrum> spectra;
+  std::vector<double> wl, val, cdf;
+  std::map<const ssol_spectrum*, int> spec_index;
+  std::vector<ssolh_medium> media;
+  std::vector<std::pair<DataKey, DataKey>> media_keys;
+  std::vector<ssolh_material> materials;
+  std::map<const ssol_material*, uint32_t> mtl_index;
+  std::vector<ssolh_shape> shapes;
+  std::map<const ssol_shape*, uint32_t> shape_index;
+  std::vector<ssolh_sshape> sshapes;
+  std::vector<ssolh_object> objects;
+  std::map<const ssol_object*, uint32_t> obj_index;
+  std::vector<unsigned> obj_depth;
+  std::vector<aabb> obj_box;  // object-space bounds of each BLAS
+  std::vector<ssolh_bvh_node> nodes;
+  std::vector<ssolh_triangle> tris;
+  std::vector<ssolh_instance> instances;
+  std::vector<uint32_t> tlas_inst;
+  std::vector<ssolh_inst_sshape> isshapes;
+  std::vector<ssolh_sample_tri> stris;
+  std::vector<ssolh_alias> alias;
+  ssolh_scene scn{};
+};
+
+int spectrum_id(Flat& f, const ssol_spectrum* s) {
+  auto it = f.spec_index.find(s);
+  if (it != f.spec_index.end()) return it->second;
+  const int id = (int)f.spectra.size();
+  ssolh_spectrum d;
+  d.offset = (uint32_t)f.wl.size();
+  d.count = (uint32_t)s->wavelengths.size();
+  f.spectra.push_back(d);
+  double acc = 0;
+  for (size_t i = 0; i < s->wavelengths.size(); ++i) {
+    if (i) acc += 0.5 * (s->intensities[i - 1] + s->intensities[i]) *
+                  (s->wavelengths[i] - s->wavelengths[i - 1]);
+    f.wl.push_back(s->wavelengths[i]);
+    f.val.push_back(s->intensities[i]);
+    f.cdf.push_back(acc);
+  }
+  f.spec_index[s] = id;
+  return id;
+}
+
+ssolh_data flat_data(Flat& f, const ssol_data& d, DataKey& key) {
+  ssolh_data o{};
+  if (d.type == SSOL_DATA_REAL) {
+    o.real = d.value.real;
+    o.spectrum = -1;
+    key = DataKey{-1, d.value.real, 0, 0};
+  } else {
+    o.real = 0;
+    o.spectrum = spectrum_id(f, d.value.spectrum);
+    key = DataKey{1, 0, d.value.spectrum->checksum[0], d.value.spectrum->checksum[1]};
+  }
+  return o;
+}
+
+bool key_eq(const DataKey& a, const DataKey& b) {
+  if (a.spectrum != b.spectrum) return false;
+  if (a.spectrum < 0) return a.real == b.real;
+  return a.c0 == b.c0 && a.c1 == b.c1;
+}
+
+// Media are identified with the reference's media_ceq rule
+uint32_t medium_id(Flat& f, const ssol_medium& m) {
+  DataKey ke, kr;
+  ssolh_medium md;
+  md.extinction = flat_data(f, m.extinction, ke);
+  md.refractive_index = flat_data(f, m.refractive_index, kr);
+  for (size_t i = 0; i < f.media.size(); ++i)
+    if (key_eq(f.media_keys[i].first, ke) && key_eq(f.media_keys[i].second, kr)) return (uint32_t)i;
+  f.media.push_back(md);
+  f.media_keys.push_back({ke, kr});
+  return (uint32_t)(f.media.size() - 1);
+}
+
+// Evaluate a shader on probe fragments; require a constant value.
+bool bake_scalar(ssol_device* dev, const ssol_material* m, ssol_shader_getter_T fn,
+                 const std::vector<double>& wls, double* out, const char* what) {
+  double first = 0;
+  bool have = false;
+  for (int k = 0; k < 3; ++k) {
+    ssol_surface_fragment fr = SSOL_SURFACE_FRAGMENT_NULL;
+    const double a = 0.3 + 0.7 * k;
+    fr.dir[0] = 0.1 * k; fr.dir[1] = -0.2; fr.dir[2] = -1;
+    fr.P[0] = 1.5 * k; fr.P[1] = -2.0 * k; fr.P[2] = 0.25 * k;
+    fr.Ng[0] = 0; fr.Ng[1] = std::sin(a); fr.Ng[2] = std::cos(a);
+    fr.Ns[0] = std::sin(a); fr.Ns[1] = 0; fr.Ns[2] = std::cos(a);
+    fr.uv[0] = 0.25 * k; fr.uv[1] = 1 - 0.25 * k;
+    fr.dPdu[0] = 1; fr.dPdv[1] = 1;
+    for (double wl : wls) {
+      double v = 0;
+      fn(dev, m->buf, wl, &fr, &v);
+      if (!have) { first = v; have = true; }
+      else if (v != first) {
+        log_error(dev, "ssol_solve: the %s shader of a material varies with the wavelength or "
+                       "the surface fragment; only constant shaders run on the GPU.\n", what);
+        return false;
+      }
+    }
+  }
+  *out = first;
+  return true;
+}
+
+bool bake_normal(ssol_device* dev, const ssol_material* m, int* from_ng) {
+  ssol_surface_fragment fr = SSOL_SURFACE_FRAGMENT_NULL;
+  fr.dir[2] = -1;
+  fr.Ng[0] = 0; fr.Ng[1] = 0.6; fr.Ng[2] = 0.8;
+  fr.Ns[0] = 0.8; fr.Ns[1] = 0; fr.Ns[2] = 0.6;
+  double v[3] = {0, 0, 0};
+  m->normal(dev, m->buf, 500, &fr, v);
+  if (v[0] == fr.Ns[0] && v[1] == fr.Ns[1] && v[2] == fr.Ns[2]) { *from_ng = 0; return true; }
+  if (v[0] == fr.Ng[0] && v[1] == fr.Ng[1] && v[2] == fr.Ng[2]) { *from_ng = 1; return true; }
+  log_error(dev, "ssol_solve: a normal shader returns neither the shading nor the geometry "
+                 "normal; it cannot be baked for the GPU.\n");
+  return false;
+}
+
+res_T material_id(Flat& f, ssol_device* dev, const ssol_material* m,
+                  const std::vector<double>& wls, uint32_t* out) {
+  auto it = f.mtl_index.find(m);
+  if (it != f.mtl_index.end()) { *out = it->second; return RES_OK; }
+  ssolh_material d{};
+  d.type = (int32_t)m->type;
+  d.medium_in = d.medium_out = -1;
+  d.slab_eta.spectrum = d.slab_absorption.spectrum = -1;
+  switch (m->type) {
+    case SSOL_MATERIAL_MIRROR:
+      if (!bake_normal(dev, m, &d.normal_from_ng)) return RES_BAD_ARG;
+      if (!bake_scalar(dev, m, m->reflectivity, wls, &d.reflectivity, "reflectivity")) return RES_BAD_ARG;
+      if (!bake_scalar(dev, m, m->roughness, wls, &d.roughness, "roughness")) return RES_BAD_ARG;
+      d.distribution = (int32_t)m->distrib;
+      break;
+    case SSOL_MATERIAL_MATTE:
+      if (!bake_normal(dev, m, &d.normal_from_ng)) return RES_BAD_ARG;
+      if (!bake_scalar(dev, m, m->reflectivity, wls, &d.reflectivity, "reflectivity")) return RES_BAD_ARG;
+      break;
+    case SSOL_MATERIAL_DIELECTRIC:
+      if (!bake_normal(dev, m, &d.normal_from_ng)) return RES_BAD_ARG;
+      d.medium_in = (int32_t)medium_id(f, m->in_medium);
+      d.medium_out = (int32_t)medium_id(f, m->out_medium);
+      break;
+    case SSOL_MATERIAL_THIN_DIELECTRIC: {
+      if (!bake_normal(dev, m, &d.normal_from_ng)) return RES_BAD_ARG;
+      d.medium_in = (int32_t)medium_id(f, m->in_medium);
+      d.medium_out = (int32_t)medium_id(f, m->out_medium);
+      DataKey k;
+      d.slab_eta = flat_data(f, m->slab_medium.refractive_index, k);
+      d.slab_absorption = flat_data(f, m->slab_medium.extinction, k);
+      d.thickness = m->thickness;
+      break;
+    }
+    case SSOL_MATERIAL_VIRTUAL: break;
+    default: return RES_BAD_ARG;
+  }
+  const uint32_t id = (uint32_t)f.materials.size();
+  f.materials.push_back(d);
+  f.mtl_index[m] = id;
+  *out = id;
+  return RES_OK;
+}
+
+void mat_inv(const double* m, double* inv) {
+  double t[9];
+  t[0] = m[4] * m[8] - m[7] * m[5];
+  t[1] = m[7] * m[2] - m[1] * m[8];
+  t[2] = m[1] * m[5] - m[4] * m[2];
+  t[3] = m[6] * m[5] - m[3] * m[8];
+  t[4] = m[0] * m[8] - m[6] * m[2];
+  t[5] = m[3] * m[2] - m[0] * m[5];
+  t[6] = m[3] * m[7] - m[6] * m[4];
+  t[7] = m[6] * m[1] - m[0] * m[7];
+  t[8] = m[0] * m[4] - m[3] * m[1];
+  const double det = m[0] * t[0] + m[3] * t[1] + m[6] * t[2];
+  for (int i = 0; i < 9; ++i) inv[i] = t[i] / det;
+}
+
+void mat_mul(const double* a, const double* b, double* out) {
+  double t[9];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r)
+      t[c * 3 + r] = a[r] * b[c * 3] + a[3 + r] * b[c * 3 + 1] + a[6 + r] * b[c * 3 + 2];
+  memcpy(out, t, sizeof(t));
+}
+
+void inflate(aabb& b, float rel, float abs_) {
+  float m = 0;
+  for (int k = 0; k < 3; ++k) m = std::max(m, std::max(std::fabs(b.lo[k]), std::fabs(b.hi[k])));
+  const float e = m * rel + abs_;
+  for (int k = 0; k < 3; ++k) { b.lo[k] -= e; b.hi[k] += e; }
+}
+
+res_T object_id(Flat& f, const ssol_object* o, uint32_t* out) {
+  auto it = f.obj_index.find(o);
+  if (it != f.obj_index.end()) { *out = it->second; return RES_OK; }
+  ssolh_object d{};
+  d.sshape_first = (uint32_t)f.sshapes.size();
+  d.sshape_count = (uint32_t)o->sshapes.size();
+  std::vector<aabb> boxes;
+  std::vector<ssolh_triangle> tris;
+  aabb all;
+  for (int k = 0; k < 3; ++k) { all.lo[k] = FLT_MAX; all.hi[k] = -FLT_MAX; }
+  for (size_t s = 0; s < o->sshapes.size(); ++s) {
+    const ssol_shape* sh = o->sshapes[s].shape;
+    for (size_t t = 0; t < sh->ids.size() / 3; ++t) {
+      ssolh_triangle tr{};
+      aabb b;
+      for (int k = 0; k < 3; ++k) { b.lo[k] = FLT_MAX; b.hi[k] = -FLT_MAX; }
+      for (int v = 0; v < 3; ++v) {
+        const float* p = &sh->pos[3 * sh->ids[3 * t + v]];
+        float* dst = v == 0 ? tr.v0 : (v == 1 ? tr.v1 : tr.v2);
+        for (int k = 0; k < 3; ++k) {
+          dst[k] = p[k];
+          b.lo[k] = std::min(b.lo[k], p[k]);
+          b.hi[k] = std::max(b.hi[k], p[k]);
+        }
+      }
+      inflate(b, 1e-6f, 1e-7f);
+      for (int k = 0; k < 3; ++k) { all.lo[k] = std::min(all.lo[k], b.lo[k]); all.hi[k] = std::max(all.hi[k], b.hi[k]); }
+      tr.sshape = (uint32_t)s;
+      tr.prim = (uint32_t)t;
+      tris.push_back(tr);
+      boxes.push_back(b);
+    }
+  }
+  std::vector<uint32_t> order;
+  const size_t node0 = f.nodes.size();
+  unsigned depth = 0;
+  d.blas_root = bvh_build(boxes, 4, f.nodes, order, &depth);
+  const uint32_t base = (uint32_t)f.tris.size();
+  for (size_t n = node0; n < f.nodes.size(); ++n) {
+    for (int c = 0; c < 2; ++c) {
+      if (f.nodes[n].count[c] != 0 && f.nodes[n].count[c] != SSOLH_BVH_EMPTY)
+        f.nodes[n].child[c] += base;
+    }
+  }
+  for (uint32_t i : order) f.tris.push_back(tris[i]);
+  if (f.tris.size() >= (1u << 28)) return RES_MEM_ERR;
+  *out = (uint32_t)f.objects.size();
+  f.objects.push_back(d);
+  f.obj_depth.push_back(depth);
+  f.obj_box.push_back(all);
+  f.obj_index[o] = *out;
+  for (const shaded_shape& ss : o->sshapes) {
+    ssolh_sshape e{};
+    auto si = f.shape_index.find(ss.shape);
+    if (si == f.shape_index.end()) {
+      ssolh_shape s{};
+      s.type = ss.shape->kind == shape_kind::mesh ? SSOLH_SHAPE_MESH : SSOLH_SHAPE_PUNCHED;
+      s.quadric = ss.shape->quadric;
+      s.ntris = (uint32_t)(ss.shape->ids.size() / 3);
+      memcpy(s.qparam, ss.shape->qparam, sizeof(s.qparam));
+      memcpy(s.qtransform, ss.shape->qtransform, sizeof(s.qtransform));
+      f.shape_index[ss.shape] = (uint32_t)f.shapes.size();
+      f.shapes.push_back(s);
+      si = f.shape_index.find(ss.shape);
+    }
+    e.shape = si->second;
+    f.sshapes.push_back(e);  // materials filled by the caller
+  }
+  return RES_OK;
+}
+
+// Vose alias table over non-negative weights
+void build_alias(const std::vector<double>& w, std::vector<ssolh_alias>& out) {
+  const size_t n = w.size();
+  out.assign(n, ssolh_alias{1.0, 0, 0});
+  double total = 0;
+  for (double x : w) total += x;
+  std::vector<double> p(n);
+  std::vector<uint32_t> small, large;
+  for (size_t i = 0; i < n; ++i) {
+    p[i] = total > 0 ? w[i] * (double)n / total : 1.0;
+    (p[i] < 1 ? small : large).push_back((uint32_t)i);
+  }
+  while (!small.empty() && !large.empty()) {
+    const uint32_t s = small.back(); small.pop_back();
+    const uint32_t l = large.back();
+    out[s].prob = p[s];
+    out[s].alias = l;
+    p[l] = (p[l] + p[s]) - 1.0;
+    if (p[l] < 1) { large.pop_back(); small.push_back(l); }
+  }
+  for (uint32_t l : large) { out[l].prob = 1.0; out[l].alias = l; }
+  for (uint32_t s : small) { out[s].prob = 1.0; out[s].alias = s; }
+}
+
+// Buie sunshape state (src/ssol_ranst_sun_dir.c:88-222)
+void buie_params(double csr, double* p) {
+  double chi;
+  if (csr > 0.145)
+    chi = -0.04419909985804843 + csr * (1.401323894233574 + csr * (-0.3639746714505299 +
+          csr * (-0.9579768560161194 + 1.1550475450828657 * csr)));
+  else if (csr > 0.035)
+    chi = 0.022652077593662934 + csr * (0.5252380349996234 + csr * (2.5484334534423887 -
+          0.8763755326550412 * csr));
+  else
+    chi = 0.004733749294807862 + csr * (4.716738065192151 + csr * (-463.506669149804 +
+          csr * (24745.88727411664 + csr * (-606122.7511711778 + 5521693.445014727 * csr))));
+  const double thetaSD = 0.00465, thetaCS = 0.0436, integralA = 9.224724736098827E-6;
+  const double k = 0.9 * std::log(13.5 * chi) * std::pow(chi, -0.3);
+  const double gamma = 2.2 * std::log(0.52 * chi) * std::pow(chi, 0.43) - 0.1;
+  const double g2 = gamma + 2.0;
+  const double ek = std::exp(k) * std::pow(1000, gamma);
+  const double integralB = ek / g2 * (std::pow(thetaCS, g2) - std::pow(thetaSD, g2));
+  const double alpha = 1 / (integralA + integralB);
+  auto pdf = [&](double th) {
+    const double phi = th < thetaSD ? std::cos(326 * th) / std::cos(308 * th) : ek * std::pow(th, gamma);
+    return alpha * phi * std::sin(th);
+  };
+  const double h1 = 1.001 * pdf(0.0038915695846209047);
+  const double h2 = pdf(thetaSD);
+  const double a1 = thetaSD * h1, a2 = (thetaCS - thetaSD) * h2;
+  p[0] = thetaSD; p[1] = thetaCS - thetaSD; p[2] = gamma; p[3] = ek;
+  p[4] = alpha; p[5] = h1; p[6] = h2; p[7] = a1 / (a1 + a2);
+}
+
+void frame(const double* d, double* m) {
+  const double ax = std::fabs(d[0]), ay = std::fabs(d[1]), az = std::fabs(d[2]);
+  double h[3] = {0, 0, 0};
+  if (ax <= ay && ax <= az) h[0] = 1; else if (ay <= az) h[1] = 1; else h[2] = 1;
+  double x[3] = {h[1] * d[2] - h[2] * d[1], h[2] * d[0] - h[0] * d[2], h[0] * d[1] - h[1] * d[0]};
+  const double n = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  for (double& v : x) v /= n;
+  const double y[3] = {d[1] * x[2] - d[2] * x[1], d[2] * x[0] - d[0] * x[2], d[0] * x[1] - d[1] * x[0]};
+  for (int k = 0; k < 3; ++k) { m[k] = x[k]; m[3 + k] = y[k]; m[6 + k] = d[k]; }
+}
+
+struct Layout {
+  std::vector<int> rcv_slot, samp_slot;
+  std::map<std::pair<uint32_t, uint32_t>, size_t> prim_first;
+  size_t nrcv = 0, nsamp = 0, nprim = 0;
+};
+
+res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
+  ssol_device* dev = scn->dev;
+  ssol_sun* sun = scn->sun;
+  // wavelengths at which shaders are probed
+  std::vector<double> wls = sun->spectrum->wavelengths;
+  if (wls.size() > 3) wls = {wls.front(), wls[wls.size() / 2], wls.back()};
+
+  // air (src/ssol_solver.c:660-664)
+  ssol_medium air = SSOL_MEDIUM_VACUUM;
+  if (scn->atmosphere) air.extinction = scn->atmosphere->extinction;
+  const uint32_t air_id = medium_id(f, air);
+
+  double S = 0, S_proxy = 0;
+  bool has_sampled = false, has_receiver = false;
+  std::vector<double> weights;
+  for (size_t i = 0; i < scn->instances.size(); ++i) {
+    ssol_instance* in = scn->instances[i];
+    for (const shaded_shape& ss : in->object->sshapes)
+      if (!ss.shape->ready) {
+        log_error(dev, "ssol_solve: a shape of the scene is not set up.\n");
+        return RES_BAD_ARG;
+      }
+    uint32_t oid;
+    res_T r = object_id(f, in->object, &oid);
+    if (r != RES_OK) return r;
+    const ssolh_object& obj = f.objects[oid];
+    for (size_t s = 0; s < in->object->sshapes.size(); ++s) {
+      uint32_t mf, mb;
+      if ((r = material_id(f, dev, in->object->sshapes[s].front, wls, &mf)) != RES_OK) return r;
+      if ((r = material_id(f, dev, in->object->sshapes[s].back, wls, &mb)) != RES_OK) return r;
+      f.sshapes[obj.sshape_first + s].mtl_front = mf;
+      f.sshapes[obj.sshape_first + s].mtl_back = mb;
+    }
+    ssolh_instance d{};
+    memcpy(d.transform, in->transform, sizeof(d.transform));
+    mat_inv(in->transform, d.inv_linear);
+    d.object = oid;
+    d.receiver_mask = (uint32_t)in->receiver_mask;
+    d.receiver = in->receiver_mask ? (int32_t)L.nrcv++ : -1;
+    d.sampled = in->sample ? (int32_t)L.nsamp++ : -1;
+    d.isshape_first = (uint32_t)f.isshapes.size();
+    d.user_id = in->id;
+    L.rcv_slot.push_back(d.receiver);
+    L.samp_slot.push_back(d.sampled);
+    if (in->receiver_mask) has_receiver = true;
+    for (size_t s = 0; s < in->object->sshapes.size(); ++s) {
+      const ssol_shape* sh = in->object->sshapes[s].shape;
+      ssolh_inst_sshape is{};
+      if (sh->kind == shape_kind::punched) {
+        mat_mul(in->transform, sh->qtransform, is.R);
+        for (int k = 0; k < 3; ++k)
+          is.T[k] = in->transform[k] * sh->qtransform[9] + in->transform[3 + k] * sh->qtransform[10] +
+                    in->transform[6 + k] * sh->qtransform[11] + in->transform[9 + k];
+        mat_inv(is.R, is.R_inv);
+      } else {
+        is.R[0] = is.R[4] = is.R[8] = 1;
+        is.R_inv[0] = is.R_inv[4] = is.R_inv[8] = 1;
+      }
+      is.prim_tally = -1;
+      if (in->receiver_mask && in->per_primitive) {
+        is.prim_tally = (int64_t)L.nprim;
+        L.prim_first[{in->id, sh->uid}] = L.nprim;
+        L.nprim += sh->ids.size() / 3;
+      }
+      f.isshapes.push_back(is);
+      if (in->sample) {
+        const std::vector<float>& sp = sh->kind == shape_kind::punched ? sh->samp_pos : sh->pos;
+        for (size_t t = 0; t < sh->ids.size() / 3; ++t) {
+          ssolh_sample_tri st{};
+          for (int v = 0; v < 3; ++v) {
+            const float* p = &sp[3 * sh->ids[3 * t + v]];
+            float* dst = v == 0 ? st.v0 : (v == 1 ? st.v1 : st.v2);
+            for (int k = 0; k < 3; ++k) dst[k] = p[k];
+          }
+          st.instance = (uint32_t)i;
+          st.sshape = (uint32_t)s;
+          st.prim = (uint32_t)t;
+          const double e0[3] = {(double)st.v1[0] - st.v0[0], (double)st.v1[1] - st.v0[1], (double)st.v1[2] - st.v0[2]};
+          const double e1[3] = {(double)st.v2[0] - st.v0[0], (double)st.v2[1] - st.v0[1], (double)st.v2[2] - st.v0[2]};
+          const double n[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2],
+                               e0[0] * e1[1] - e0[1] * e1[0]};
+          weights.push_back(0.5 * std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]));
+          f.stris.push_back(st);
+        }
+      }
+    }
+    if (in->sample) {
+      has_sampled = true;
+      S += in->object->rt_area;
+      S_proxy += in->object->samp_area;
+    }
+    f.instances.push_back(d);
+  }
+  if (!has_sampled || f.stris.empty()) {
+    log_error(dev, "No solstice instance to sample.\n");
+    return RES_BAD_ARG;
+  }
+  if (!has_receiver) log_warning(dev, "No receiver is defined.\n");
+  build_alias(weights, f.alias);
+
+  // TLAS over the world boxes of the instances
+  std::vector<aabb> boxes(f.instances.size());
+  unsigned max_blas = 0;
+  for (size_t i = 0; i < f.instances.size(); ++i) {
+    const ssolh_instance& in = f.instances[i];
+    const aabb& ob = f.obj_box[in.object];
+    max_blas = std::max(max_blas, f.obj_depth[in.object]);
+    aabb& b = boxes[i];
+    for (int k = 0; k < 3; ++k) { b.lo[k] = FLT_MAX; b.hi[k] = -FLT_MAX; }
+    for (int c = 0; c < 8; ++c) {
+      const double p[3] = {(c & 1) ? ob.hi[0] : ob.lo[0], (c & 2) ? ob.hi[1] : ob.lo[1],
+                           (c & 4) ? ob.hi[2] : ob.lo[2]};
+      for (int k = 0; k < 3; ++k) {
+        const double w = in.transform[k] * p[0] + in.transform[3 + k] * p[1] +
+                         in.transform[6 + k] * p[2] + in.transform[9 + k];
+        b.lo[k] = std::min(b.lo[k], (float)w);
+        b.hi[k] = std::max(b.hi[k], (float)w);
+      }
+    }
+    if (ob.lo[0] > ob.hi[0]) {  // empty object: box that no ray reaches
+      for (int k = 0; k < 3; ++k) { b.lo[k] = 0; b.hi[k] = 0; }
+    }
+    inflate(b, 2e-6f, 1e-6f);
+  }
+  unsigned tdepth = 0;
+  f.scn.tlas_root = bvh_build(boxes, 1, f.nodes, f.tlas_inst, &tdepth);
+  if (tdepth + max_blas + 2 > (unsigned)GPU_STACK) {
+    log_error(dev, "ssol_solve: BVH too deep for the GPU traversal stack (%u + %u levels).\n",
+              tdepth, max_blas);
+    return RES_MEM_ERR;
+  }
+
+  // sun (src/ssol_sun.c:214-279)
+  ssolh_sun& su = f.scn.sun;
+  memset(&su, 0, sizeof(su));
+  for (int k = 0; k < 3; ++k) su.dir[k] = sun->direction[k];
+  frame(su.dir, su.basis);
+  su.dni = sun->dni;
+  su.wl_spectrum = spectrum_id(f, sun->spectrum);
+  switch (sun->kind) {
+    case sun_kind::directional: su.type = SSOLH_SUN_DIRECTIONAL; break;
+    case sun_kind::pillbox:
+      if (!(sun->param > 0)) return RES_BAD_ARG;
+      su.type = SSOLH_SUN_PILLBOX;
+      su.param[0] = std::sin(sun->param) * std::sin(sun->param);
+      break;
+    case sun_kind::gaussian:
+      if (!(sun->param > 0)) return RES_BAD_ARG;
+      su.type = SSOLH_SUN_GAUSSIAN;
+      su.param[0] = sun->param;
+      break;
+    case sun_kind::buie:
+      if (sun->param < 1e-6 || sun->param > 0.849) return RES_BAD_ARG;
+      su.type = SSOLH_SUN_BUIE;
+      buie_params(sun->param, su.param);
+      break;
+  }
+  {
+    const ssolh_spectrum& sp = f.spectra[su.wl_spectrum];
+    su.wl_cdf_total = sp.count ? f.cdf[sp.offset + sp.count - 1] : 0;
+  }
+
+  ssolh_scene& s = f.scn;
+  s.nspectra = (uint32_t)f.spectra.size();
+  s.spectra_len = (uint32_t)f.wl.size();
+  s.spectra = f.spectra.data();
+  s.spectra_wl = f.wl.data();
+  s.spectra_val = f.val.data();
+  s.spectra_cdf = f.cdf.data();
+  s.nmedia = (uint32_t)f.media.size();
+  s.air_medium = air_id;
+  s.media = f.media.data();
+  s.nmaterials = (uint32_t)f.materials.size();
+  s.materials = f.materials.data();
+  s.nshapes = (uint32_t)f.shapes.size();
+  s.shapes = f.shapes.data();
+  s.nsshapes = (uint32_t)f.sshapes.size();
+  s.sshapes = f.sshapes.data();
+  s.nobjects = (uint32_t)f.objects.size();
+  s.objects = f.objects.data();
+  s.nnodes = (uint32_t)f.nodes.size();
+  s.nodes = f.nodes.data();
+  s.ntriangles = (uint32_t)f.tris.size();
+  s.triangles = f.tris.data();
+  s.ninstances = (uint32_t)f.instances.size();
+  s.instances = f.instances.data();
+  s.tlas_instances = f.tlas_inst.data();
+  s.ninst_sshapes = (uint32_t)f.isshapes.size();
+  s.inst_sshapes = f.isshapes.data();
+  s.nsample_tris = (uint32_t)f.stris.size();
+  s.sample_tris = f.stris.data();
+  s.sample_alias = f.alias.data();
+  s.sampled_area = S;
+  s.sampled_area_proxy = S_proxy;
+  s.nreceivers = (uint32_t)L.nrcv;
+  s.nsampled = (uint32_t)L.nsamp;
+  s.nprim_tallies = L.nprim;
+  return RES_OK;
+}
+
+void mc_result(double w, double w2, double N, ssol_mc_result* r) {
+  r->E = w / N;
+  r->V = w2 / N - r->E * r->E;
+  r->V = r->V > 0 ? r->V : 0;
+  r->SE = std::sqrt(r->V / N);
+}
+
+void estimator_release(ssol_estimator* e) {
+  if (e->rng) ssp_rng_ref_put(e->rng);
+  ssol_device* d = e->dev;
+  delete e;
+  device_ref_put_(d);
+}
+
+void fill_receiver(const double* base, double N, ssol_mc_receiver* r) {
+  ssol_mc_result* fields[10] = {&r->incoming_flux, &r->incoming_if_no_atm_loss,
+                                &r->incoming_if_no_field_loss, &r->incoming_lost_in_field,
+                                &r->incoming_lost_in_atmosphere, &r->absorbed_flux,
+                                &r->absorbed_if_no_atm_loss, &r->absorbed_if_no_field_loss,
+                                &r->absorbed_lost_in_field, &r->absorbed_lost_in_atmosphere};
+  for (int f = 0; f < 10; ++f) mc_result(base[2 * f], base[2 * f + 1], N, fields[f]);
+}
+
+}  // namespace
+
+extern "C" {
+
+res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t N,
+                 const size_t max_failed, const struct ssol_path_tracker* tracker,
+                 struct ssol_estimator** out) {
+  if (!scn || !rng || !N || !out) return RES_BAD_ARG;
+  *out = nullptr;
+  if (N > (size_t)INT64_MAX || max_failed > (size_t)INT64_MAX) return RES_BAD_ARG;
+  ssol_device* dev = scn->dev;
+  if (!scn->sun) { log_error(dev, "ssol_solve: no sun attached.\n"); return RES_BAD_ARG; }
+  if (!scn->sun->spectrum || scn->sun->spectrum->wavelengths.empty()) {
+    log_error(dev, "ssol_solve: sun's spectrum undefined.\n");
+    return RES_BAD_ARG;
+  }
+  if (scn->sun->dni <= 0) { log_error(dev, "ssol_solve: sun's DNI undefined.\n"); return RES_BAD_ARG; }
+  if (tracker) {
+    log_error(dev, "ssol_solve: path tracking is not available on the GPU solver.\n");
+    return RES_BAD_ARG;
+  }
+  Flat f;
+  Layout L;
+  res_T r = flatten(scn, f, L);
+  if (r != RES_OK) return r;
+
+  if (!dev->gpu) {
+    const int e = ssolh_create(0, &dev->gpu);
+    if (e) {
+      log_error(dev, "ssol_solve: %s\n", ssolh_last_error());
+      fprintf(stderr, "ssol_solve: GPU unavailable: %s\n", ssolh_last_error());
+      dev->gpu = nullptr;
+      return RES_UNKNOWN_ERR;
+    }
+  }
+  int e = ssolh_scene_upload(dev->gpu, &f.scn);
+  if (e) {
+    fprintf(stderr, "ssol_solve: scene upload failed: %s\n", ssolh_last_error());
+    return e == SSOLH_ERR_ARG ? RES_BAD_ARG : RES_MEM_ERR;
+  }
+  ssolh_solve_args a{};
+  a.realisations = N;
+  a.max_failed = max_failed;
+  a.rng_key = rng->key;
+  a.rng_counter = rng->counter;
+  a.precision = dev->precision == SSOL_PRECISION_FP32 ? SSOLH_FP32 : SSOLH_FP64;
+  a.path_slots = (uint32_t)std::min<size_t>(dev->path_slots, UINT32_MAX);
+  a.pending_entries = 4;
+  e = ssolh_solve(dev->gpu, &a);
+  if (e) {
+    fprintf(stderr, "ssol_solve: GPU solve failed: %s\n", ssolh_last_error());
+    return e == SSOLH_ERR_ARG ? RES_BAD_ARG : RES_UNKNOWN_ERR;
+  }
+
+  ssol_estimator* est = new ssol_estimator();
+  est->dev = dev;
+  ref_get(dev);
+  est->nrcv = L.nrcv;
+  est->nsamp = L.nsamp;
+  est->rcv.assign(L.nrcv * 40, 0);
+  est->samp.assign(L.nsamp * 4, 0);
+  est->samp_count.assign(L.nsamp, 0);
+  est->sxr.assign(L.nsamp * L.nrcv * 40, 0);
+  est->prim.assign(L.nprim * 40, 0);
+  ssolh_tallies t{};
+  t.receivers = est->rcv.data();
+  t.sampled = est->samp.data();
+  t.sampled_count = (uint64_t*)est->samp_count.data();
+  t.sampled_x_receivers = est->sxr.data();
+  t.primitives = est->prim.data();
+  e = ssolh_tallies_download(dev->gpu, &t);
+  if (e) {
+    fprintf(stderr, "ssol_solve: tally download failed: %s\n", ssolh_last_error());
+    estimator_release(est);
+    return RES_UNKNOWN_ERR;
+  }
+  memcpy(est->global, t.global, sizeof(est->global));
+  est->realisation_count = t.realisation_count;
+  est->failed_count = t.failed_count;
+  est->sampled_area = f.scn.sampled_area;
+  for (size_t i = 0; i < scn->instances.size(); ++i) {
+    const ssol_instance* in = scn->instances[i];
+    if (L.rcv_slot[i] >= 0) est->rcv_slot[in->id] = L.rcv_slot[i];
+    if (L.samp_slot[i] >= 0) est->samp_slot[in->id] = L.samp_slot[i];
+  }
+  est->prim_first = L.prim_first;
+  est->stats.segments = t.segments;
+  est->stats.paths = N;
+  est->stats.pending_spills = t.pending_spills;
+  est->stats.kernel_ms = t.kernel_ms;
+  est->stats.trace_ms = t.trace_ms;
+  est->stats.shade_ms = t.shade_ms;
+  est->stats.iterations = t.iterations;
+  // RNG state after the run: the next realisations' counters
+  ssp_rng* nr = new ssp_rng();
+  nr->type = rng->type;
+  nr->key = rng->key;
+  nr->counter = rng->counter + N;
+  est->rng = nr;
+  *out = est;
+  if (t.aborted) {
+    log_error(dev, "Too many unexpected radiative paths.\n");
+    return RES_BAD_OP;
+  }
+  return RES_OK;
+}
+
+res_T ssol_estimator_ref_get(struct ssol_estimator* e) {
+  if (!e) return RES_BAD_ARG;
+  ref_get(e);
+  return RES_OK;
+}
+
+res_T ssol_estimator_ref_put(struct ssol_estimator* e) {
+  if (!e) return RES_BAD_ARG;
+  if (--e->ref == 0) estimator_release(e);
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_mc_global(struct ssol_estimator* e, struct ssol_mc_global* g) {
+  if (!e || !g) return RES_BAD_ARG;
+  const double N = (double)e->realisation_count;
+  ssol_mc_result* fields[6] = {&g->cos_factor, &g->absorbed_by_receivers, &g->shadowed,
+                               &g->missing, &g->extinguished_by_atmosphere, &g->other_absorbed};
+  for (int i = 0; i < 6; ++i) mc_result(e->global[i][0], e->global[i][1], N, fields[i]);
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_mc_receiver(struct ssol_estimator* e, const struct ssol_instance* inst,
+                                     const enum ssol_side_flag side, struct ssol_mc_receiver* rcv) {
+  if (!e || !inst || !rcv || !(inst->receiver_mask & (int)side)) return RES_BAD_ARG;
+  memset(rcv, 0, sizeof(*rcv));
+  auto it = e->rcv_slot.find(inst->id);
+  if (it == e->rcv_slot.end()) return RES_OK;
+  const int s = side == SSOL_FRONT ? 0 : 1;
+  fill_receiver(&e->rcv[((size_t)it->second * 2 + s) * 20], (double)e->realisation_count, rcv);
+  e->refs.push_back(std::unique_ptr<ssol_estimator::rcv_ref>(
+      new ssol_estimator::rcv_ref{e, it->second, s, inst}));
+  rcv->mc__ = e->refs.back().get();
+  rcv->N__ = e->realisation_count;
+  rcv->instance__ = inst;
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_mc_sampled_x_receiver(struct ssol_estimator* e,
+                                               const struct ssol_instance* samp,
+                                               const struct ssol_instance* ri,
+                                               const enum ssol_side_flag side,
+                                               struct ssol_mc_receiver* rcv) {
+  if (!e || !samp || !ri || !rcv || (side != SSOL_BACK && side != SSOL_FRONT) || !samp->sample ||
+      !(ri->receiver_mask & (int)side))
+    return RES_BAD_ARG;
+  memset(rcv, 0, sizeof(*rcv));
+  auto si = e->samp_slot.find(samp->id);
+  if (si == e->samp_slot.end()) return RES_BAD_ARG;
+  auto it = e->rcv_slot.find(ri->id);
+  if (it == e->rcv_slot.end()) return RES_OK;
+  const int s = side == SSOL_FRONT ? 0 : 1;
+  const size_t key = ((size_t)si->second * e->nrcv + (size_t)it->second) * 2 + (size_t)s;
+  fill_receiver(&e->sxr[key * 20], (double)e->realisation_count, rcv);
+  e->refs.push_back(std::unique_ptr<ssol_estimator::rcv_ref>(
+      new ssol_estimator::rcv_ref{e, -1, s, ri}));
+  rcv->mc__ = e->refs.back().get();
+  rcv->N__ = e->samp_count[si->second];
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_realisation_count(const struct ssol_estimator* e, size_t* n) {
+  if (!e || !n) return RES_BAD_ARG;
+  *n = e->realisation_count;
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_failed_count(const struct ssol_estimator* e, size_t* n) {
+  if (!e || !n) return RES_BAD_ARG;
+  *n = e->failed_count;
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_sampled_area(const struct ssol_estimator* e, double* area) {
+  if (!e || !area) return RES_BAD_ARG;
+  *area = e->sampled_area;
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_sampled_count(const struct ssol_estimator* e, size_t* n) {
+  if (!e || !n) return RES_BAD_ARG;
+  *n = e->nsamp;
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_mc_sampled(struct ssol_estimator* e, const struct ssol_instance* inst,
+                                    struct ssol_mc_sampled* s) {
+  if (!e || !inst || !s) return RES_BAD_ARG;
+  auto it = e->samp_slot.find(inst->id);
+  if (it == e->samp_slot.end()) return RES_BAD_ARG;
+  const double* b = &e->samp[(size_t)it->second * 4];
+  s->nb_samples = e->samp_count[it->second];
+  mc_result(b[0], b[1], (double)s->nb_samples, &s->cos_factor);
+  mc_result(b[2], b[3], (double)e->realisation_count, &s->shadowed);
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_rng_state(const struct ssol_estimator* e, const struct ssp_rng** r) {
+  if (!e || !r) return RES_BAD_ARG;
+  *r = e->rng;
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_tracked_paths_count(const struct ssol_estimator* e, size_t* n) {
+  if (!e || !n) return RES_BAD_ARG;
+  *n = 0;
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_tracked_path(const struct ssol_estimator* e, const size_t,
+                                      struct ssol_path* p) {
+  (void)e; (void)p;
+  return RES_BAD_ARG;  // no tracked paths are recorded
+}
+
+res_T ssol_path_get_vertices_count(const struct ssol_path* p, size_t* n) {
+  if (!p || !n || !p->path__) return RES_BAD_ARG;
+  return RES_BAD_ARG;
+}
+
+res_T ssol_path_get_vertex(const struct ssol_path* p, const size_t, struct ssol_path_vertex* v) {
+  if (!p || !v) return RES_BAD_ARG;
+  return RES_BAD_ARG;
+}
+
+res_T ssol_path_get_type(const struct ssol_path* p, enum ssol_path_type* t) {
+  if (!p || !t) return RES_BAD_ARG;
+  return RES_BAD_ARG;
+}
+
+res_T ssol_mc_receiver_get_mc_shape(struct ssol_mc_receiver* rcv, const struct ssol_shape* shape,
+                                    struct ssol_mc_shape* mc) {
+  if (!rcv || !shape || !mc || !rcv->mc__ || !rcv->instance__) return RES_BAD_ARG;
+  const ssol_instance* inst = rcv->instance__;
+  bool owns = false;
+  for (const shaded_shape& ss : inst->object->sshapes) owns |= ss.shape == shape;
+  if (!owns) return RES_BAD_ARG;
+  auto* ref = (ssol_estimator::rcv_ref*)rcv->mc__;
+  ssol_estimator* e = ref->est;
+  mc->N__ = rcv->N__;
+  mc->shape__ = shape;
+  mc->mc__ = nullptr;
+  auto it = e->prim_first.find({inst->id, shape->uid});
+  if (it != e->prim_first.end() && ref->slot >= 0) {
+    e->shape_refs.push_back(std::unique_ptr<ssol_estimator::shape_ref>(
+        new ssol_estimator::shape_ref{e, it->second * 2 + (size_t)ref->side, shape}));
+    mc->mc__ = e->shape_refs.back().get();
+  }
+  return RES_OK;
+}
+
+res_T ssol_mc_shape_get_mc_primitive(struct ssol_mc_shape* sh, const unsigned i,
+                                     struct ssol_mc_primitive* prim) {
+  if (!sh || !prim || !sh->shape__) return RES_BAD_ARG;
+  const ssol_shape* shape = sh->shape__;
+  if (i >= shape->ids.size() / 3) return RES_BAD_ARG;
+  ssol_mc_result* fields[10] = {&prim->incoming_flux, &prim->incoming_if_no_atm_loss,
+                                &prim->incoming_if_no_field_loss, &prim->incoming_lost_in_field,
+                                &prim->incoming_lost_in_atmosphere, &prim->absorbed_flux,
+                                &prim->absorbed_if_no_atm_loss, &prim->absorbed_if_no_field_loss,
+                                &prim->absorbed_lost_in_field, &prim->absorbed_lost_in_atmosphere};
+  if (!sh->mc__) {
+    for (auto* f : fields) *f = SSOL_MC_RESULT_NULL;
+    return RES_OK;
+  }
+  auto* ref = (ssol_estimator::shape_ref*)sh->mc__;
+  // first = slot*2 + side of triangle 0; triangle i is (slot + i)*2 + side
+  const size_t key = ref->first + 2 * (size_t)i;
+  const double* b = &ref->est->prim[key * 20];
+  const float* v0 = &shape->pos[3 * shape->ids[3 * i]];
+  const float* v1 = &shape->pos[3 * shape->ids[3 * i + 1]];
+  const float* v2 = &shape->pos[3 * shape->ids[3 * i + 2]];
+  const double e0[3] = {(double)v1[0] - v0[0], (double)v1[1] - v0[1], (double)v1[2] - v0[2]};
+  const double e1[3] = {(double)v2[0] - v0[0], (double)v2[1] - v0[1], (double)v2[2] - v0[2]};
+  const double n[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2],
+                       e0[0] * e1[1] - e0[1] * e1[0]};
+  const double area = 0.5 * std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int f = 0; f < 10; ++f) {
+    mc_result(b[2 * f], b[2 * f + 1], (double)sh->N__, fields[f]);
+    fields[f]->E /= area;
+    fields[f]->V /= area * area;
+    fields[f]->SE /= area;
+  }
+  return RES_OK;
+}
+
+res_T ssol_estimator_get_solve_stats(const struct ssol_estimator* e, struct ssol_solve_stats* s) {
+  if (!e || !s) return RES_BAD_ARG;
+  *s = e->stats;
+  return RES_OK;
+}
+
+}  // extern "C"
