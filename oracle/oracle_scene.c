@@ -742,23 +742,40 @@ ray_box(const double org[3], const double inv[3], double tmin, double tmax,
   return 1;
 }
 
-/* Moller-Trumbore; returns the distance or -1 */
+/* Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013): edge
+ * functions in a ray-aligned shear frame, so that a ray through a shared
+ * edge or vertex hits at least one of the triangles. Returns the distance or
+ * -1. */
 static double
 ray_tri(const double org[3], const double dir[3], const struct owtri* t)
 {
-  double p[3], q[3], s[3], det, inv, u, v, d;
-  ov_cross(p, dir, t->e2);
-  det = ov_dot(t->e1, p);
+  int kx, ky, kz;
+  double Sx, Sy, Sz, A[3], B[3], C[3], Ax, Ay, Bx, By, Cx, Cy, U, V, W, det, T;
+  const double ax = fabs(dir[0]), ay = fabs(dir[1]), az = fabs(dir[2]);
+  int i;
+  kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  kx = (kz + 1) % 3;
+  ky = (kx + 1) % 3;
+  if(dir[kz] < 0) { const int tmp = kx; kx = ky; ky = tmp; }
+  Sx = dir[kx] / dir[kz];
+  Sy = dir[ky] / dir[kz];
+  Sz = 1.0 / dir[kz];
+  for(i = 0; i < 3; ++i) {
+    A[i] = t->v0[i] - org[i];
+    B[i] = t->v0[i] + t->e1[i] - org[i];
+    C[i] = t->v0[i] + t->e2[i] - org[i];
+  }
+  Ax = A[kx] - Sx * A[kz]; Ay = A[ky] - Sy * A[kz];
+  Bx = B[kx] - Sx * B[kz]; By = B[ky] - Sy * B[kz];
+  Cx = C[kx] - Sx * C[kz]; Cy = C[ky] - Sy * C[kz];
+  U = Cx * By - Cy * Bx;
+  V = Ax * Cy - Ay * Cx;
+  W = Bx * Ay - By * Ax;
+  if((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return -1;
+  det = U + V + W;
   if(det == 0) return -1;
-  inv = 1 / det;
-  s[0] = org[0] - t->v0[0]; s[1] = org[1] - t->v0[1]; s[2] = org[2] - t->v0[2];
-  u = ov_dot(s, p) * inv;
-  if(u < 0 || u > 1) return -1;
-  ov_cross(q, s, t->e1);
-  v = ov_dot(dir, q) * inv;
-  if(v < 0 || u + v > 1) return -1;
-  d = ov_dot(t->e2, q) * inv;
-  return d;
+  T = U * Sz * A[kz] + V * Sz * B[kz] + W * Sz * C[kz];
+  return T / det;
 }
 
 void

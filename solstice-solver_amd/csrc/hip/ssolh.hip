@@ -237,19 +237,48 @@ __device__ __forceinline__ bool box_hit(const FRay& r, const float* lo, const fl
   return tn <= tf;
 }
 
-__device__ __forceinline__ double ray_triangle(d3 o, d3 d, d3 v0, d3 v1, d3 v2) {
-  const d3 e1 = v1 - v0, e2 = v2 - v0;
-  const d3 p = cross(d, e2);
-  const double det = dot(e1, p);
+// Ray in the shear frame of the watertight ray/triangle test (Woop,
+// Benthin, Wald, JCGT 2013): computed once per ray (and per instance).
+struct Shear {
+  int kx, ky, kz;
+  double Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ Shear make_shear(d3 d) {
+  const double a[3] = {fabs(d.x), fabs(d.y), fabs(d.z)};
+  const double v[3] = {d.x, d.y, d.z};
+  Shear s;
+  s.kz = a[0] > a[1] ? (a[0] > a[2] ? 0 : 2) : (a[1] > a[2] ? 1 : 2);
+  s.kx = (s.kz + 1) % 3;
+  s.ky = (s.kx + 1) % 3;
+  if (v[s.kz] < 0) { const int t = s.kx; s.kx = s.ky; s.ky = t; }
+  s.Sx = v[s.kx] / v[s.kz];
+  s.Sy = v[s.ky] / v[s.kz];
+  s.Sz = 1.0 / v[s.kz];
+  return s;
+}
+
+__device__ __forceinline__ double comp(d3 p, int k) { return k == 0 ? p.x : (k == 1 ? p.y : p.z); }
+
+// Watertight: a ray through a shared edge or vertex hits at least one of
+// the triangles. The edge functions of a shared edge must come out as exact
+// negations of each other in the two triangles, so no product here may be
+// fused into an FMA. Returns the distance or -1.
+__device__ __forceinline__ double ray_triangle(d3 o, const Shear& s, d3 v0, d3 v1, d3 v2) {
+#pragma clang fp contract(off)
+  const d3 A = v0 - o, B = v1 - o, C = v2 - o;
+  const double Az = comp(A, s.kz), Bz = comp(B, s.kz), Cz = comp(C, s.kz);
+  const double Ax = comp(A, s.kx) - s.Sx * Az, Ay = comp(A, s.ky) - s.Sy * Az;
+  const double Bx = comp(B, s.kx) - s.Sx * Bz, By = comp(B, s.ky) - s.Sy * Bz;
+  const double Cx = comp(C, s.kx) - s.Sx * Cz, Cy = comp(C, s.ky) - s.Sy * Cz;
+  const double U = Cx * By - Cy * Bx;
+  const double V = Ax * Cy - Ay * Cx;
+  const double W = Bx * Ay - By * Ax;
+  if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return -1;
+  const double det = U + V + W;
   if (det == 0) return -1;
-  const double inv = 1.0 / det;
-  const d3 s = o - v0;
-  const double u = dot(s, p) * inv;
-  if (u < 0 || u > 1) return -1;
-  const d3 q = cross(s, e1);
-  const double v = dot(d, q) * inv;
-  if (v < 0 || u + v > 1) return -1;
-  return dot(e2, q) * inv;
+  const double T = U * s.Sz * Az + V * s.Sz * Bz + W * s.Sz * Cz;
+  return T / det;
 }
 
 struct TraceState {
@@ -337,6 +366,7 @@ k_trace(DScene S, const RayRec* __restrict__ rays, HitRec* __restrict__ hits,
   const FRay fw = make_fray(ow, dw);
   FRay fr = fw;
   d3 o = ow, d = dw;        // current-level ray (double)
+  Shear sh = make_shear(dw);
   uint32_t cur_inst = NONE;  // instance whose BLAS is being traversed
   ssolh_instance inst{};
   const float ftmin = (float)r.tmin;
@@ -361,13 +391,14 @@ k_trace(DScene S, const RayRec* __restrict__ rays, HitRec* __restrict__ hits,
         o = mul(inst.inv_linear, ow - T);
         d = mul(inst.inv_linear, dw);
         fr = make_fray(o, d);
+        sh = make_shear(d);
         if (sp < STACK) { stack[sp * BLOCK] = SENTINEL; ++sp; }
         ref = S.objects[inst.object].blas_root;
         continue;
       }
       for (uint32_t k = 0; k < cnt; ++k) {
         const ssolh_triangle tri = S.tris[first + k];
-        const double t = ray_triangle(o, d, ld3(tri.v0), ld3(tri.v1), ld3(tri.v2));
+        const double t = ray_triangle(o, sh, ld3(tri.v0), ld3(tri.v1), ld3(tri.v2));
         if (!(t >= r.tmin && t < ts.tbest)) continue;
         d3 ng;
         if (!filter(S, r, ow, dw, cur_inst, inst, tri, t, ts, ng)) continue;
