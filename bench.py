@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the flux integrator (ssol_solve) on one MI355X.
+
+Workload: the 10,000-heliostat beam-down field (8x8-facet flat heliostats,
+128 triangles each, km-scale layout; hyperboloid secondary; 3 mm thin glass
+cover over a 64x64-cell receiver; atmosphere 0.1/km; Russian roulette on),
+synthetic and seeded. A step is one ssol_solve of `--realisations` paths on
+the scene already resident in HBM (the first warm-up solve flattens and
+uploads it; later solves reuse it). The metric is Mpaths/s; the time to reach
+0.1% relative standard error on absorbed_by_receivers is measured with
+RNG-chained batches merged like src/test_ssol_solver6.c:219-235.
+
+Prints one JSON line. --gpus N > 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "solstice-solver_amd", "python")]
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per traced segment (FP64 path state):
+#  trace kernel: reads the 56 B of path state it needs, writes a 32 B hit record
+#  shade kernel: reads path state (96 B) and hit (32 B), writes path state
+#                (96 B) and the compacted queue index (8 B)
+BYTES_TRACE_FP64 = 56 + 32
+BYTES_SHADE_FP64 = 96 + 32 + 96 + 8
+BYTES_TRACE_FP32 = 40 + 16
+BYTES_SHADE_FP32 = 64 + 16 + 64 + 8
+
+
+def merge(a, b):
+    """Merge two (E, SE, N) estimates (src/test_ssol_solver6.c:219-235)."""
+    (E1, SE1, N1), (E2, SE2, N2) = a, b
+    sw = E1 * N1 + E2 * N2
+    sw2 = (SE1 * SE1 * N1 + E1 * E1) * N1 + (SE2 * SE2 * N2 + E2 * E2) * N2
+    N = N1 + N2
+    E = sw / N
+    V = max(sw2 / N - E * E, 0.0)
+    return E, math.sqrt(V / N), N
+
+
+def time_to_rel_se(lib, batch, target=1e-3, seed=7, max_paths=2_000_000_000):
+    import ssol_ctypes as L
+    t0 = time.perf_counter()
+    rng = L.Rng(seed, 0)
+    acc = None
+    state = rng
+    keep = []
+    while True:
+        est = lib.solve(batch, rng=state)
+        g = est.global_()["absorbed_by_receivers"]
+        cur = (g[0], g[2], est.realisation_count)
+        acc = cur if acc is None else merge(acc, cur)
+        keep.append(est)          # the next RNG state is borrowed from it
+        state = est.rng_state
+        if len(keep) > 2:
+            keep.pop(0)
+        if acc[0] > 0 and acc[1] / acc[0] <= target:
+            break
+        if acc[2] >= max_paths:
+            break
+    return time.perf_counter() - t0, acc
+
+
+def cpu_baseline(spec, seconds: float, threads: int):
+    """The oracle (FP64 OpenMP restatement of the reference solver) on the
+    host cores, on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ctypes as O
+    o = O.Oracle(spec, track_sampled_x_receiver=False)
+    chunk = 50_000
+    o.solve(2000, seed=99, nthreads=threads)  # warm-up (page-in, thread pool)
+    n = 0
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < seconds:
+        o.solve(chunk, seed=1000 + k, nthreads=threads)
+        n += chunk
+        k += 1
+    dt = time.perf_counter() - t0
+    return n / dt / 1e6, n, dt
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--realisations", type=int, default=100_000_000)
+    ap.add_argument("--config", choices=("c1", "c2", "c3"), default="c3")
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
+    ap.add_argument("--path-slots", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ttt", action="store_true", help="skip the time-to-0.1%% run")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the flux integrator on one GPU; --gpus {args.gpus} "
+              "is not supported (the realisations shard with no exchange but a tally "
+              "all-reduce; see DESIGN.md).", file=sys.stderr)
+        return 2
+
+    import ssol_scene as S
+    import ssol_ctypes as L
+
+    spec = {"c1": S.config1, "c2": S.config2, "c3": S.config3}[args.config]()
+    workload = {"c1": "config1_flat_mirror", "c2": "config2_100_paraboloids",
+                "c3": "config3_10k_heliostat_beam_down"}[args.config]
+    print(f"[bench] building {spec.name} ({len(spec.instances)} instances)", file=sys.stderr,
+          flush=True)
+    lib = L.Scene(spec, precision=args.precision, path_slots=args.path_slots)
+    B = args.realisations
+
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        est = lib.solve(B, seed=100 + w)
+        print(f"[bench] warmup {w}: {time.perf_counter() - t:.3f} s", file=sys.stderr, flush=True)
+
+    times, segs, trace_ms, shade_ms, iters = [], 0, 0.0, 0.0, 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        t = time.perf_counter()
+        est = lib.solve(B, seed=k)
+        times.append(time.perf_counter() - t)
+        st = est.stats
+        segs += st.segments
+        trace_ms += st.trace_ms
+        shade_ms += st.shade_ms
+        iters += st.iterations
+        print(f"[bench] step {k}: {times[-1]:.3f} s, {B / times[-1] / 1e6:.1f} Mpaths/s, "
+              f"{st.segments / B:.3f} segments/path, trace {st.trace_ms:.1f} ms, "
+              f"shade {st.shade_ms:.1f} ms, {st.iterations} iterations", file=sys.stderr, flush=True)
+    total = time.perf_counter() - t0
+    value = args.steps * B / total / 1e6
+    g = est.global_()
+
+    # roofline of the dominant kernel, from in-stream HIP events
+    fp32 = args.precision == "fp32"
+    if trace_ms >= shade_ms:
+        kernel, ms, per_seg = "k_trace", trace_ms, (BYTES_TRACE_FP32 if fp32 else BYTES_TRACE_FP64)
+    else:
+        kernel, ms, per_seg = "k_shade", shade_ms, (BYTES_SHADE_FP32 if fp32 else BYTES_SHADE_FP64)
+    achieved = per_seg * segs / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": None, "bytes_per_segment": per_seg,
+                "avg_launch_ms": round(ms / iters, 4), "segments_per_path": round(segs / (B * args.steps), 4)}
+
+    out = {
+        "metric": "Mpaths/s + time to 0.1% rel. std-err on absorbed flux, 10k-heliostat field",
+        "value": round(value, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000 * total / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if not fp32 else "f32",
+        "data": "synthetic",
+        "config": {"workload": workload, "realisations_per_step": B,
+                   "instances": len(spec.instances), "precision": args.precision},
+        "roofline": roofline,
+        "absorbed_by_receivers": {"E": g["absorbed_by_receivers"][0],
+                                  "SE": g["absorbed_by_receivers"][2]},
+    }
+
+    if not args.no_ttt:
+        dt, acc = time_to_rel_se(lib, batch=min(B, 4_000_000))
+        out["time_to_0.1pct_rel_se_s"] = round(dt, 4)
+        out["paths_to_0.1pct_rel_se"] = acc[2]
+        print(f"[bench] 0.1% rel. SE on absorbed after {acc[2]} paths in {dt:.3f} s",
+              file=sys.stderr, flush=True)
+
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        mps, n, dt = cpu_baseline(spec, args.cpu_seconds, threads)
+        out["cpu_baseline"] = {"value": round(mps, 4), "unit": "Mpaths/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{n} paths of the same workload in {dt:.1f} s "
+                                         "(oracle: FP64 OpenMP restatement of the reference "
+                                         "solver with its own SAH BVH)"}
+        if "time_to_0.1pct_rel_se_s" in out:
+            out["cpu_baseline"]["time_to_0.1pct_rel_se_s_est"] = round(
+                out["paths_to_0.1pct_rel_se"] / (mps * 1e6), 1)
+        print(f"[bench] CPU baseline: {mps:.4f} Mpaths/s on {threads} threads", file=sys.stderr,
+              flush=True)
+
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

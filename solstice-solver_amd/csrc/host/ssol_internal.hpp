@@ -24,7 +24,10 @@ void log_warning(const ssol_device* dev, const char* fmt, ...);
 // Reference counting shared by every handle
 struct ssol_refcounted {
   int ref = 1;
+  uint64_t gen = 0;  // bumped by every setter; part of the solve cache key
 };
+
+struct solve_cache;  // flattened scene of the last solve (ssol_solve.cpp)
 
 struct ssol_device : ssol_refcounted {
   int verbose = 0;
@@ -32,6 +35,7 @@ struct ssol_device : ssol_refcounted {
   enum ssol_precision precision = SSOL_PRECISION_FP64;
   size_t path_slots = 0;
   ssolh_context* gpu = nullptr;  // created at the first solve
+  solve_cache* cache = nullptr;
 };
 
 struct ssol_spectrum : ssol_refcounted {
@@ -173,6 +177,8 @@ namespace ssol {
 template <typename T> inline void ref_get(T* o) { ++o->ref; }
 
 void device_ref_put_(ssol_device* d);
+void solve_cache_free(ssol_device* d);
+uint64_t next_gen();  // monotonic stamp for ssol_refcounted::gen
 
 // Punched-surface tessellation (ssol_shape.cpp)
 res_T punched_setup(ssol_shape* shape, const ssol_punched_surface* ps);

@@ -40,8 +40,14 @@ uint64_t fnv64(const void* data, size_t n) {
   return h;
 }
 
+uint64_t next_gen() {
+  static uint64_t g = 0;
+  return ++g;
+}
+
 void device_ref_put_(ssol_device* d) {
   if (--d->ref == 0) {
+    solve_cache_free(d);
     if (d->gpu) ssolh_destroy(d->gpu);
     delete d;
   }
@@ -179,6 +185,7 @@ res_T ssol_spectrum_setup(struct ssol_spectrum* s,
     }
     cur = s->wavelengths[i];
   }
+  s->gen = next_gen();
   s->checksum[0] = fnv64(s->wavelengths.data(), n * sizeof(double));
   s->checksum[1] = fnv64(s->intensities.data(), n * sizeof(double));
   return RES_OK;
@@ -323,6 +330,7 @@ res_T ssol_sun_ref_put(struct ssol_sun* s) {
 }
 
 res_T ssol_sun_set_direction(struct ssol_sun* s, const double dir[3]) {
+  if (s) s->gen = next_gen();
   if (!s || !dir) return RES_BAD_ARG;
   const double n = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
   if (n == 0) return RES_BAD_ARG;
@@ -337,6 +345,7 @@ res_T ssol_sun_get_direction(const struct ssol_sun* s, double dir[3]) {
 }
 
 res_T ssol_sun_set_dni(struct ssol_sun* s, const double dni) {
+  if (s) s->gen = next_gen();
   if (!s || dni <= 0) return RES_BAD_ARG;
   s->dni = dni;
   return RES_OK;
@@ -349,6 +358,7 @@ res_T ssol_sun_get_dni(const struct ssol_sun* s, double* dni) {
 }
 
 res_T ssol_sun_set_spectrum(struct ssol_sun* s, struct ssol_spectrum* sp) {
+  if (s) s->gen = next_gen();
   if (!s || !sp) return RES_BAD_ARG;
   if (sp == s->spectrum) return RES_OK;
   if (s->spectrum) spectrum_put(s->spectrum);
@@ -358,18 +368,21 @@ res_T ssol_sun_set_spectrum(struct ssol_sun* s, struct ssol_spectrum* sp) {
 }
 
 res_T ssol_sun_pillbox_set_half_angle(struct ssol_sun* s, const double a) {
+  if (s) s->gen = next_gen();
   if (!s || a <= 0 || a > M_PI * 0.5 || s->kind != sun_kind::pillbox) return RES_BAD_ARG;
   s->param = a;
   return RES_OK;
 }
 
 res_T ssol_sun_gaussian_set_std_dev(struct ssol_sun* s, const double sd) {
+  if (s) s->gen = next_gen();
   if (!s || sd <= 0 || s->kind != sun_kind::gaussian) return RES_BAD_ARG;
   s->param = sd;
   return RES_OK;
 }
 
 res_T ssol_sun_set_buie_param(struct ssol_sun* s, const double r) {
+  if (s) s->gen = next_gen();
   if (!s || r <= 0 || r >= 1 || s->kind != sun_kind::buie) return RES_BAD_ARG;
   s->param = r;
   return RES_OK;
@@ -407,6 +420,7 @@ res_T ssol_atmosphere_set_extinction(struct ssol_atmosphere* a, struct ssol_data
     return RES_BAD_ARG;
   }
   ssol_data_copy(&a->extinction, ext);
+  a->gen = next_gen();
   return RES_OK;
 }
 
@@ -470,6 +484,7 @@ res_T ssol_material_ref_put(struct ssol_material* m) {
 }
 
 res_T ssol_material_set_param_buffer(struct ssol_material* m, struct ssol_param_buffer* b) {
+  if (m) m->gen = next_gen();
   if (!m || !b) return RES_BAD_ARG;
   ref_get(b);
   if (m->buf) ssol_param_buffer_ref_put(m->buf);
@@ -479,6 +494,7 @@ res_T ssol_material_set_param_buffer(struct ssol_material* m, struct ssol_param_
 
 res_T ssol_dielectric_setup(struct ssol_material* m, const struct ssol_dielectric_shader* sh,
                             const struct ssol_medium* out_med, const struct ssol_medium* in_med) {
+  if (m) m->gen = next_gen();
   if (!m || m->type != SSOL_MATERIAL_DIELECTRIC || !sh || !sh->normal || !check_medium(out_med) ||
       !check_medium(in_med))
     return RES_BAD_ARG;
@@ -490,6 +506,7 @@ res_T ssol_dielectric_setup(struct ssol_material* m, const struct ssol_dielectri
 
 res_T ssol_mirror_setup(struct ssol_material* m, const struct ssol_mirror_shader* sh,
                         const enum ssol_microfacet_distribution distrib) {
+  if (m) m->gen = next_gen();
   if (!m || m->type != SSOL_MATERIAL_MIRROR || !sh || !sh->normal || !sh->reflectivity ||
       !sh->roughness || (unsigned)distrib >= SSOL_MICROFACET_DISTRIBUTIONS_COUNT__)
     return RES_BAD_ARG;
@@ -501,6 +518,7 @@ res_T ssol_mirror_setup(struct ssol_material* m, const struct ssol_mirror_shader
 }
 
 res_T ssol_matte_setup(struct ssol_material* m, const struct ssol_matte_shader* sh) {
+  if (m) m->gen = next_gen();
   if (!m || m->type != SSOL_MATERIAL_MATTE || !sh || !sh->normal || !sh->reflectivity)
     return RES_BAD_ARG;
   m->normal = sh->normal;
@@ -512,6 +530,7 @@ res_T ssol_thin_dielectric_setup(struct ssol_material* m,
                                  const struct ssol_thin_dielectric_shader* sh,
                                  const struct ssol_medium* out_med,
                                  const struct ssol_medium* slab, const double thickness) {
+  if (m) m->gen = next_gen();
   if (!m || m->type != SSOL_MATERIAL_THIN_DIELECTRIC || !sh || !sh->normal ||
       !check_medium(out_med) || !check_medium(slab) || thickness < 0)
     return RES_BAD_ARG;
@@ -909,6 +928,7 @@ res_T ssol_mesh_setup(struct ssol_shape* s, const unsigned ntris,
                       void (*get_indices)(const unsigned, unsigned[3], void*),
                       const unsigned nverts, const struct ssol_vertex_data attribs[],
                       const unsigned nattribs, void* data) {
+  if (s) s->gen = next_gen();
   if (!s || s->kind != shape_kind::mesh || !get_indices || !ntris || !nverts || !attribs ||
       !nattribs)
     return RES_BAD_ARG;
@@ -956,6 +976,7 @@ res_T ssol_mesh_setup(struct ssol_shape* s, const unsigned ntris,
 }
 
 res_T ssol_punched_surface_setup(struct ssol_shape* s, const struct ssol_punched_surface* ps) {
+  if (s) s->gen = next_gen();
   if (!s || s->kind != shape_kind::punched) return RES_BAD_ARG;
   return punched_setup(s, ps);
 }

@@ -367,6 +367,66 @@ struct Layout {
   size_t nrcv = 0, nsamp = 0, nprim = 0;
 };
 
+void push_data_sig(std::vector<uint64_t>& sig, const ssol_data& d) {
+  sig.push_back((uint64_t)d.type);
+  if (d.type == SSOL_DATA_REAL) {
+    uint64_t b;
+    memcpy(&b, &d.value.real, 8);
+    sig.push_back(b);
+  } else {
+    sig.push_back((uint64_t)(uintptr_t)d.value.spectrum);
+    sig.push_back(d.value.spectrum->gen);
+  }
+}
+
+void push_medium_sig(std::vector<uint64_t>& sig, const ssol_medium& m) {
+  push_data_sig(sig, m.extinction);
+  push_data_sig(sig, m.refractive_index);
+}
+
+// Everything flatten() reads, so that an unchanged scene is not flattened
+// and uploaded again
+std::vector<uint64_t> scene_signature(const ssol_scene* scn) {
+  std::vector<uint64_t> sig;
+  auto P = [&](const void* p) { sig.push_back((uint64_t)(uintptr_t)p); };
+  P(scn);
+  P(scn->sun);
+  sig.push_back(scn->sun->gen);
+  P(scn->sun->spectrum);
+  sig.push_back(scn->sun->spectrum->gen);
+  P(scn->atmosphere);
+  if (scn->atmosphere) {
+    sig.push_back(scn->atmosphere->gen);
+    push_data_sig(sig, scn->atmosphere->extinction);
+  }
+  for (const ssol_instance* in : scn->instances) {
+    P(in);
+    sig.push_back(in->id);
+    for (int k = 0; k < 12; ++k) {
+      uint64_t b;
+      memcpy(&b, &in->transform[k], 8);
+      sig.push_back(b);
+    }
+    sig.push_back((uint64_t)in->receiver_mask);
+    sig.push_back((uint64_t)in->per_primitive);
+    sig.push_back((uint64_t)in->sample);
+    P(in->object);
+    sig.push_back(in->object->sshapes.size());
+    for (const shaded_shape& ss : in->object->sshapes) {
+      P(ss.shape);
+      sig.push_back(ss.shape->gen);
+      for (const ssol_material* m : {ss.front, ss.back}) {
+        P(m);
+        sig.push_back(m->gen);
+        push_medium_sig(sig, m->in_medium);
+        push_medium_sig(sig, m->out_medium);
+        push_medium_sig(sig, m->slab_medium);
+      }
+    }
+  }
+  return sig;
+}
+
 res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
   ssol_device* dev = scn->dev;
   ssol_sun* sun = scn->sun;
@@ -592,6 +652,19 @@ void fill_receiver(const double* base, double N, ssol_mc_receiver* r) {
 
 }  // namespace
 
+struct solve_cache {
+  std::vector<uint64_t> sig;
+  Flat flat;
+  Layout layout;
+};
+
+namespace ssol {
+void solve_cache_free(ssol_device* d) {
+  delete d->cache;
+  d->cache = nullptr;
+}
+}  // namespace ssol
+
 extern "C" {
 
 res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t N,
@@ -611,10 +684,19 @@ res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t
     log_error(dev, "ssol_solve: path tracking is not available on the GPU solver.\n");
     return RES_BAD_ARG;
   }
-  Flat f;
-  Layout L;
-  res_T r = flatten(scn, f, L);
-  if (r != RES_OK) return r;
+  std::vector<uint64_t> sig = scene_signature(scn);
+  const bool cached = dev->cache && dev->cache->sig == sig;
+  if (!cached) {
+    solve_cache_free(dev);
+    dev->cache = new solve_cache();
+    res_T r = flatten(scn, dev->cache->flat, dev->cache->layout);
+    if (r != RES_OK) {
+      solve_cache_free(dev);
+      return r;
+    }
+  }
+  Flat& f = dev->cache->flat;
+  Layout& L = dev->cache->layout;
 
   if (!dev->gpu) {
     const int e = ssolh_create(0, &dev->gpu);
@@ -625,10 +707,15 @@ res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t
       return RES_UNKNOWN_ERR;
     }
   }
-  int e = ssolh_scene_upload(dev->gpu, &f.scn);
-  if (e) {
-    fprintf(stderr, "ssol_solve: scene upload failed: %s\n", ssolh_last_error());
-    return e == SSOLH_ERR_ARG ? RES_BAD_ARG : RES_MEM_ERR;
+  int e = 0;
+  if (!cached) {
+    e = ssolh_scene_upload(dev->gpu, &f.scn);
+    if (e) {
+      fprintf(stderr, "ssol_solve: scene upload failed: %s\n", ssolh_last_error());
+      solve_cache_free(dev);
+      return e == SSOLH_ERR_ARG ? RES_BAD_ARG : RES_MEM_ERR;
+    }
+    dev->cache->sig.swap(sig);
   }
   ssolh_solve_args a{};
   a.realisations = N;

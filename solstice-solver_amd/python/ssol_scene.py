@@ -373,7 +373,7 @@ def config3(seed: int = 0, n_helio: int = 10000, r_min: float = 100.0,
     hyp = Punched(Q_HYPERBOL, (100.0, 500.0), list(IDENTITY),
                   [rect_carving(160.0, 160.0)])
     sec_mtl = mirror(0.95)
-    sec_obj = Object([(hyp, sec_mtl, virtual())])
+    sec_obj = Object([(hyp, sec_mtl, matte(0.0))])
     sec = Instance(sec_obj, transform(None, (0.0, 0.0, H_vertex)), sample=0)
     glass = thin_dielectric(VACUUM, Medium(Data(0.0), Data(1.5)), 3e-3)
     cover_obj = Object([(square_mesh(6.0), glass, glass)])
