@@ -69,6 +69,35 @@ def time_to_rel_se(lib, batch, target=1e-3, seed=7, max_paths=2_000_000_000):
     return time.perf_counter() - t0, acc
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=5):
+    """GB/s of a plain device copy (read + write bytes), the achievable HBM
+    yardstick; one dispatch moves nbytes in and nbytes out."""
+    import ctypes as C
+    import ssol_ctypes as L
+    hip = L.lib_hip()
+    ctx = C.c_void_p()
+    if hip.ssolh_create(0, C.byref(ctx)):
+        raise RuntimeError("ssolh_create failed")
+    gbs = C.c_double()
+    hip.ssolh_copy_bandwidth.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+    rc = hip.ssolh_copy_bandwidth(ctx, nbytes, reps, C.byref(gbs))
+    hip.ssolh_destroy(ctx)
+    if rc:
+        raise RuntimeError(f"ssolh_copy_bandwidth returned {rc}")
+    return gbs.value
+
+
+def load_traffic(kernel, workload):
+    """HBM bytes per segment of `kernel` on `workload` from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    plain copy), or None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def cpu_baseline(spec, seconds: float, threads: int):
     """The oracle (FP64 OpenMP restatement of the reference solver) on the
     host cores, on a bounded sample of the same workload."""
@@ -99,6 +128,7 @@ def main(argv=None):
     ap.add_argument("--path-slots", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth yardstick")
     ap.add_argument("--no-ttt", action="store_true", help="skip the time-to-0.1%% run")
     args = ap.parse_args(argv)
 
@@ -149,10 +179,18 @@ def main(argv=None):
     else:
         kernel, ms, per_seg = "k_shade", shade_ms, (BYTES_SHADE_FP32 if fp32 else BYTES_SHADE_FP64)
     achieved = per_seg * segs / (ms * 1e-3) / 1e9
+    seg_per_launch = segs / iters
+    tr = load_traffic(kernel, workload + ("_fp32" if fp32 else ""))
+    traffic = None if tr is None else round(tr["hbm_bytes_per_segment"] * seg_per_launch)
     roofline = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "bytes_per_segment": per_seg,
-                "avg_launch_ms": round(ms / iters, 4), "segments_per_path": round(segs / (B * args.steps), 4)}
+                "traffic": traffic, "algorithmic_bytes_per_launch": round(per_seg * seg_per_launch),
+                "bytes_per_segment": per_seg, "avg_launch_ms": round(ms / iters, 4),
+                "segments_per_path": round(segs / (B * args.steps), 4),
+                "trace_ms_per_step": round(trace_ms / args.steps, 3),
+                "shade_ms_per_step": round(shade_ms / args.steps, 3)}
+    if not args.no_copy:
+        roofline["copy_gbs"] = round(copy_bandwidth(), 1)
 
     out = {
         "metric": "Mpaths/s + time to 0.1% rel. std-err on absorbed flux, 10k-heliostat field",

@@ -316,6 +316,11 @@ SSOLH_API int ssolh_test_threefry
   (ssolh_context* ctx, uint32_t n, const uint64_t* ctr, const uint64_t* key,
    uint64_t* out);
 
+/* Bandwidth of a plain device copy of `bytes` bytes repeated `reps` times
+ * (read + write bytes over the HIP-event time), in GB/s. Calibrates the HBM
+ * byte counters of the profiler; no counterpart in the reference. */
+SSOLH_API int ssolh_copy_bandwidth(ssolh_context* ctx, size_t bytes, int reps, double* gbs);
+
 #ifdef __cplusplus
 }
 #endif

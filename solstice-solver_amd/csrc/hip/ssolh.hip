@@ -977,6 +977,16 @@ __global__ void k_test_sun(ssolh_sun sun, uint32_t n, const double* u, double* o
   out[3 * i] = d.x; out[3 * i + 1] = d.y; out[3 * i + 2] = d.z;
 }
 
+// Plain device-to-device copy: reads and writes exactly 16 B per element.
+// Used to calibrate the HBM byte counters and as the achievable-bandwidth
+// yardstick for the path-tracing kernels.
+__global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ a, uint4* __restrict__ b,
+                                             size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 __global__ void k_test_threefry(uint32_t n, const uint64_t* ctr, const uint64_t* key, uint64_t* out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1352,6 +1362,31 @@ SSOLH_API int ssolh_test_threefry(ssolh_context* c, uint32_t n, const uint64_t* 
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(out, d_o, n * 32, hipMemcpyDeviceToHost));
   (void)hipFree(d_c); (void)hipFree(d_k); (void)hipFree(d_o);
+  return 0;
+}
+
+SSOLH_API int ssolh_copy_bandwidth(ssolh_context* c, size_t bytes, int reps, double* gbs) {
+  if (!c || bytes < 16 || reps < 1 || !gbs) return SSOLH_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->device));
+  const size_t n = bytes / 16;
+  uint4 *a, *b;
+  HIP_TRY(hipMalloc(&a, n * 16));
+  HIP_TRY(hipMalloc(&b, n * 16));
+  HIP_TRY(hipMemsetAsync(a, 0x3c, n * 16, c->stream));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0)); HIP_TRY(hipEventCreate(&e1));
+  const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  k_copy<<<grid, 256, 0, c->stream>>>(a, b, n);  // warm
+  HIP_TRY(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r) k_copy<<<grid, 256, 0, c->stream>>>(a, b, n);
+  HIP_TRY(hipEventRecord(e1, c->stream));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  *gbs = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+  (void)hipFree(a); (void)hipFree(b);
   return 0;
 }
 

@@ -95,6 +95,94 @@ def solver1_mono(n, backend, seed=0):
     return r
 
 
+def solver2(n, backend, seed=0):
+    """src/test_ssol_solver2.c:179-204 (mesh heliostats, punched secondary:
+    the target gets the whole beam, SE 0)"""
+    scn, h = R.solver2(True)
+    r = R.Run(backend, scn, n, seed=seed)
+    assert r.realisation_count == n
+    c = math.cos(math.pi / 4)
+    g = r.global_()
+    assert eq_eps(g["shadowed"][0], 0, 1e-4)
+    assert eq_eps(g["missing"][0], 4 * DNI * c, 1e-4), g["missing"]
+    assert eq_eps(g["cos_factor"][0], c, 1e-4)
+    with_back = r.receiver_or_none(h["heliostat1"], S.BACK)
+    assert with_back is None
+    r.receiver(h["secondary"])
+    rc = r.receiver(h["target"])
+    assert eq_eps(rc["incoming_flux"][0], 4 * DNI * c, 1e-8), rc["incoming_flux"]
+    assert eq_eps(rc["incoming_flux"][2], 0, 1e-4)
+    return r
+
+
+def solver2b(n, backend, seed=0):
+    """src/test_ssol_solver2b.c:183-206 (punched heliostats, half-size target)"""
+    scn, h = R.solver2(False)
+    r = R.Run(backend, scn, n, seed=seed)
+    assert r.realisation_count == n
+    c = math.cos(math.pi / 4)
+    m = 2 * DNI * c
+    std = math.sqrt(((4 * DNI * c) ** 2 / 2 - (2 * DNI * c) ** 2) / n)
+    g = r.global_()
+    assert eq_eps(g["shadowed"][0], 0, 1e-4)
+    assert eq_eps(g["missing"][0], 4 * DNI * c, 1e-4), g["missing"]
+    assert eq_eps(g["cos_factor"][0], c, 1e-4)
+    rc = r.receiver(h["target"])
+    assert eq_eps(rc["incoming_flux"][0], m, 2 * std), rc["incoming_flux"]
+    assert eq_eps(rc["incoming_flux"][2], std, 1e-3), (rc["incoming_flux"], std)
+    return r
+
+
+def solver3(n, backend, seed=0):
+    """src/test_ssol_solver3.c:138-163 (20x20 punched plane; area 400)"""
+    scn, h = R.solver3()
+    r = R.Run(backend, scn, n, seed=seed)
+    assert r.realisation_count == n and r.failed_count == 0
+    c = math.cos(math.pi / 4)
+    m = 4 * DNI * c
+    std = math.sqrt(((400 * DNI * c) ** 2 / 100 - (4 * DNI * c) ** 2) / n)
+    g = r.global_()
+    assert eq_eps(g["shadowed"][0], 0, 1e-4)
+    assert eq_eps(g["missing"][0], 400 * DNI * c, 1e-4), g["missing"]
+    assert eq_eps(g["cos_factor"][0], c, 1e-4)
+    rc = r.receiver(h["target"])
+    assert eq_eps(rc["incoming_flux"][0], m, 2 * std), rc["incoming_flux"]
+    assert eq_eps(rc["incoming_flux"][2], std, 10), (rc["incoming_flux"], std)
+    assert abs(r.sampled_area - 400) <= 2.220446049250313e-16 * 400
+    return r
+
+
+def solver5(n, backend, seed=0):
+    """src/test_ssol_solver5.c:138-159 (parabolic cylinder focal line: the
+    whole 400 m2 aperture flux, 1e-8)"""
+    scn, h = R.solver5()
+    r = R.Run(backend, scn, n, seed=seed)
+    assert r.realisation_count == n
+    g = r.global_()
+    assert eq_eps(g["shadowed"][0], 0, 1e-4)
+    assert eq_eps(g["missing"][0], 400 * DNI, 1e-4), g["missing"]
+    rc = r.receiver(h["target"])
+    assert eq_eps(rc["incoming_flux"][0], 400 * DNI, 1e-8), rc["incoming_flux"]
+    assert eq_eps(rc["incoming_flux"][2], 0, 1e-4)
+    return r
+
+
+def solver8(n, backend, seed=0):
+    """src/test_ssol_solver8.c:141-160 (y-carved parabolic cylinder under an
+    oblique sun: projected area sqrt(2) * 10 * 4)"""
+    scn, h = R.solver8()
+    r = R.Run(backend, scn, n, seed=seed)
+    assert r.realisation_count == n and r.failed_count == 0
+    area = math.sqrt(2) * 10 * 4
+    g = r.global_()
+    assert eq_eps(g["shadowed"][0], 0, 1e-4)
+    assert eq_eps(g["missing"][0], area * DNI, 3 * g["missing"][2]), g["missing"]
+    rc = r.receiver(h["target"])
+    assert eq_eps(rc["incoming_flux"][0], area * DNI, 2 * rc["incoming_flux"][2]), \
+        rc["incoming_flux"]
+    return r
+
+
 def solver4(n, backend, seed=0):
     """src/test_ssol_solver4.c:146-172 (focus of a paraboloid: both stacked
     virtual targets receive the whole 400 m2 aperture flux, +-0.01 W)"""

@@ -71,6 +71,82 @@ def solver1(variant: str):
     return scn, {"heliostat": helio, "secondary": secondary, "target": target, "square": square}
 
 
+# src/test_ssol_solver2.c, src/test_ssol_solver2b.c ------------------------------
+def solver2(mesh_heliostats: bool):
+    """Two 1x2 m heliostats side by side, a flipped 2x2 m punched-plane
+    secondary at (2, 0, 2) and a virtual target at x = 4. solver2: mesh
+    heliostats and a 2x2 mesh target; solver2b (mesh_heliostats False):
+    punched-plane heliostats and a 1x2 mesh target that sees half the beam."""
+    m_mtl = S.mirror(1.0)
+    v_mtl = S.virtual()
+    rect = S.rect_mesh(np.float32(-0.5), np.float32(0.5), np.float32(-1), np.float32(1))
+    if mesh_heliostats:
+        h_shape = rect
+        t_shape = S.square_mesh(1.0)
+    else:
+        h_shape = S.Punched(S.Q_PLANE, (0, 0), list(S.IDENTITY), [S.rect_carving(0.5, 1)])
+        t_shape = rect
+    sq = S.Punched(S.Q_PLANE, (0, 0), list(S.IDENTITY), [S.rect_carving(1, 1)])
+    m_obj = S.Object([(h_shape, m_mtl, m_mtl)])
+    h1 = S.Instance(m_obj, t_move(-0.5), receiver_mask=S.FRONT)
+    h2 = S.Instance(m_obj, t_move(0.5), receiver_mask=S.FRONT)
+    secondary = S.Instance(S.Object([(sq, m_mtl, m_mtl)]), t_pitch(PI, 2, 0, 2),
+                           receiver_mask=S.FRONT, sample=0)
+    target = S.Instance(S.Object([(t_shape, v_mtl, v_mtl)]), t_move(4),
+                        receiver_mask=S.FRONT, sample=0)
+    sun = S.Sun("directional", (1, 0, -1), 1000.0, spectrum3())
+    name = "solver2" if mesh_heliostats else "solver2b"
+    return (S.Scene([h1, h2, secondary, target], sun, name=name),
+            {"heliostat1": h1, "secondary": secondary, "target": target})
+
+
+# src/test_ssol_solver3.c -------------------------------------------------------
+def solver3():
+    """A 20x20 m punched-plane heliostat under a 2x2 m virtual target flipped
+    at z = 2."""
+    quad = S.Punched(S.Q_PLANE, (0, 0), list(S.IDENTITY), [S.rect_carving(10, 10)])
+    m_mtl = S.mirror(1.0)
+    v_mtl = S.virtual()
+    helio = S.Instance(S.Object([(quad, m_mtl, m_mtl)]), S.IDENTITY, receiver_mask=S.FRONT)
+    target = S.Instance(S.Object([(S.square_mesh(1.0), v_mtl, v_mtl)]), t_pitch(PI, 0, 0, 2),
+                        receiver_mask=S.FRONT, sample=0)
+    sun = S.Sun("directional", (1, 0, -1), 1000.0, spectrum3())
+    return S.Scene([helio, target], sun, name="solver3"), {"heliostat": helio, "target": target}
+
+
+# src/test_ssol_solver5.c -------------------------------------------------------
+def solver5():
+    """Parabolic cylinder (focal 10) carved to 20x20 m; a 20x0.2 m virtual
+    target on the focal line."""
+    f = 10.0
+    quad = S.Punched(S.Q_PCYLINDER, (f, 0.0), list(S.IDENTITY), [S.rect_carving(10, 10)])
+    m_mtl = S.mirror(1.0)
+    v_mtl = S.virtual()
+    helio = S.Instance(S.Object([(quad, m_mtl, m_mtl)]), S.IDENTITY)
+    rect = S.rect_mesh(np.float32(-10), np.float32(10), np.float32(-0.1), np.float32(0.1))
+    target = S.Instance(S.Object([(rect, v_mtl, v_mtl)]), t_pitch(PI, 0, 0, f),
+                        receiver_mask=S.FRONT, sample=0)
+    sun = S.Sun("directional", (0, 0, -1), 1000.0, spectrum3())
+    return S.Scene([helio, target], sun, name="solver5"), {"target": target}
+
+
+# src/test_ssol_solver8.c -------------------------------------------------------
+def solver8():
+    """Parabolic cylinder (focal 1) carved to x in [-5, 5], y in [0, 4]
+    under an oblique sun (0, 1, -1); a 20x20 m virtual target flipped at
+    z = 4."""
+    quad = S.Punched(S.Q_PCYLINDER, (1.0, 0.0), list(S.IDENTITY),
+                     [S.Carving(S.AND, np.array([[-5.0, 0.0], [-5.0, 4.0], [5.0, 4.0],
+                                                 [5.0, 0.0]]))])
+    m_mtl = S.mirror(1.0)
+    v_mtl = S.virtual()
+    helio = S.Instance(S.Object([(quad, m_mtl, m_mtl)]), S.IDENTITY)
+    target = S.Instance(S.Object([(S.square_mesh(10.0), v_mtl, v_mtl)]), t_pitch(PI, 0, 0, 4),
+                        receiver_mask=S.FRONT, sample=0)
+    sun = S.Sun("directional", (0, 1, -1), 1000.0, spectrum3())
+    return S.Scene([helio, target], sun, name="solver8"), {"target": target}
+
+
 # src/test_ssol_solver4.c -------------------------------------------------------
 def solver4():
     square = S.square_mesh(0.1)
@@ -197,6 +273,14 @@ class Run:
 
     def receiver(self, inst, side=S.FRONT):
         return self.e.receiver(inst, side)
+
+    def receiver_or_none(self, inst, side=S.FRONT):
+        """None where the reference getter returns RES_BAD_ARG (a side that is
+        not in the instance's receiver mask)."""
+        try:
+            return self.e.receiver(inst, side)
+        except (ValueError, RuntimeError):
+            return None
 
     def sampled(self, inst):
         return self.e.sampled(inst)

@@ -33,6 +33,26 @@ def test_solver1_monochromatic_spectral_extinction(gpu):
     K.solver1_mono(10000, "gpu")
 
 
+def test_solver2_mesh_heliostats(gpu):
+    K.solver2(10000, "gpu")
+
+
+def test_solver2b_punched_heliostats(gpu):
+    K.solver2b(50000, "gpu")
+
+
+def test_solver3_punched_plane(gpu):
+    K.solver3(20000, "gpu")
+
+
+def test_solver5_parabolic_cylinder(gpu):
+    K.solver5(10000, "gpu")
+
+
+def test_solver8_carved_parabolic_cylinder(gpu):
+    K.solver8(100000, "gpu")
+
+
 def test_solver4_paraboloid_focus(gpu):
     K.solver4(100000, "gpu")
 
@@ -111,7 +131,9 @@ def test_solver1_vs_oracle(gpu, variant):
     compare(R.solver1(variant)[0], 200000)
 
 
-@pytest.mark.parametrize("builder", [R.solver4, R.solver6, R.solver7, R.solver9])
+@pytest.mark.parametrize("builder", [R.solver4, R.solver6, R.solver7, R.solver9, R.solver3,
+                                     R.solver5, R.solver8, lambda: R.solver2(True),
+                                     lambda: R.solver2(False)])
 def test_reference_scene_vs_oracle(gpu, builder):
     compare(builder()[0], 200000)
 

@@ -25,6 +25,26 @@ def test_solver1_monochromatic_spectral_extinction():
     K.solver1_mono(10000, "oracle")
 
 
+def test_solver2_mesh_heliostats():
+    K.solver2(10000, "oracle")
+
+
+def test_solver2b_punched_heliostats():
+    K.solver2b(50000, "oracle")
+
+
+def test_solver3_punched_plane():
+    K.solver3(20000, "oracle")
+
+
+def test_solver5_parabolic_cylinder():
+    K.solver5(10000, "oracle")
+
+
+def test_solver8_carved_parabolic_cylinder():
+    K.solver8(100000, "oracle")
+
+
 def test_solver4_paraboloid_focus():
     K.solver4(100000, "oracle")
 
