@@ -309,9 +309,18 @@ SSOLH_API int ssolh_test_quadric_hits
   (ssolh_context* ctx, int quadric, const double qparam[4], uint32_t n,
    const double* org, const double* dir, const double* hint,
    double* t, double* normal);
+/* Sun directions from fixed uniforms: sample i draws from u[i*nu ...]
+ * (0.5 once its nu values are used up); used[i] = uniforms consumed */
 SSOLH_API int ssolh_test_sun_dirs
   (ssolh_context* ctx, const struct ssolh_sun* sun, uint32_t n,
-   const double* u, /* [n][2] */ double* dir);
+   const double* u, uint32_t nu, double* dir, uint32_t* used);
+/* BSDF samples from fixed uniforms (same convention as above): wi points
+ * away from the surface on the normal's side; lobe 1 reflect, 2 transmit */
+SSOLH_API int ssolh_test_bsdf
+  (ssolh_context* ctx, const struct ssolh_material* m, double eta_out,
+   double eta_in, double slab_eta, double slab_absorption, uint32_t n,
+   const double* wi, const double* normal, const double* u, uint32_t nu,
+   double* wo, double* weight, int* lobe);
 SSOLH_API int ssolh_test_threefry
   (ssolh_context* ctx, uint32_t n, const uint64_t* ctr, const uint64_t* key,
    uint64_t* out);

@@ -191,6 +191,7 @@ int oracle_sun_sample
   (int type, double param, const double dir[3], const double* u, int nu,
    double out[3]);
 double oracle_buie_pdf_theta(double csr, double theta);
+void oracle_buie_params(double csr, double out[8]);
 double oracle_spectrum_interpolate
   (const double* wl, const double* val, size_t n, double wavelength);
 double oracle_piecewise_linear_sample

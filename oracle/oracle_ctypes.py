@@ -130,6 +130,7 @@ def _declare(L):
     L.oracle_basis.argtypes = [R3, C.c_double * 9]
     L.oracle_sun_sample.restype = C.c_int
     L.oracle_sun_sample.argtypes = [C.c_int, C.c_double, R3, C.POINTER(C.c_double), C.c_int, R3]
+    L.oracle_buie_params.argtypes = [C.c_double, C.c_double * 8]
     L.oracle_buie_pdf_theta.restype = C.c_double
     L.oracle_buie_pdf_theta.argtypes = [C.c_double, C.c_double]
     L.oracle_spectrum_interpolate.restype = C.c_double
@@ -381,6 +382,12 @@ def sun_sample(kind, param, d, u):
     out = R3()
     used = lib().oracle_sun_sample(kind, param, R3(*d), _dptr(u), len(u), out)
     return np.array(tuple(out)), used
+
+
+def buie_params(csr):
+    out = (C.c_double * 8)()
+    lib().oracle_buie_params(csr, out)
+    return list(out)
 
 
 def buie_pdf_theta(csr, theta):

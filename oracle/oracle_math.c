@@ -333,6 +333,17 @@ obuie_pdf(const struct obuie* s, double theta)
   return s->alpha * phi * sin(theta);
 }
 
+/* The 8 sampler constants in the device order {thetaSD, dThetaCSSD, gamma,
+ * e^k 1000^gamma, alpha, h1, h2, p_rect1}, for fixed-input GPU tests */
+void
+oracle_buie_params(double csr, double out[8])
+{
+  struct obuie s;
+  obuie_setup(csr, &s);
+  out[0] = s.thetaSD; out[1] = s.dThetaCSSD; out[2] = s.gamma; out[3] = s.ek1000g;
+  out[4] = s.alpha; out[5] = s.h1; out[6] = s.h2; out[7] = s.p1;
+}
+
 double
 oracle_buie_pdf_theta(double csr, double theta)
 {

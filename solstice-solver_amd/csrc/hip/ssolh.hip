@@ -1,25 +1,30 @@
 // ssolh -- wavefront flux integrator on one MI355X (gfx950).
 //
-// Replaces the reference's OpenMP realisation loop (src/ssol_solver.c:703-732
-// running trace_radiative_path, :873-1127) by a persistent wavefront of path
-// slots in HBM:
+// Replaces the reference's OpenMP realisation loop (src/ssol_solver.c:
+// 1223-1248 running trace_radiative_path, :873-1127) by a wavefront of path
+// slots in HBM. Every per-path record (path state, next ray, last hit) is
+// indexed by its slot; the work list of an iteration is the list of slots
+// that have a ray to trace, kept in increasing slot order so that a wave
+// reads and writes contiguous runs of every record.
 //
 //   k_generate  fills the slots with new realisations (start point, sun
-//               direction, wavelength, weights) and queues their shadow rays;
-//   k_trace     one thread per queued ray: two-level BVH traversal in FP32
+//               direction, wavelength, weights) and writes their shadow rays;
+//   k_trace     one thread per listed slot: two-level BVH traversal in FP32
 //               boxes with the reference's any-hit filter inline
 //               (src/ssol_scene.c:438-536) and the FP64 quadric refinement
 //               (src/ssol_shape.c:1449-1494);
-//   k_shade     one thread per ray result: the tail of the reference loop
-//               iteration that traced it (miss, segment extinction tallies,
-//               Russian roulette, point update) and the head of the next one
-//               (extinction, BSDF or virtual pass-through, receiver tallies);
-//               finished realisations flush their pending tallies with
-//               wave-aggregated FP64 atomics and the slot is refilled with a
-//               new realisation in place; live rays are compacted into the
-//               next queue with one atomic per wave.
+//   k_shade     one thread per listed slot: the tail of the reference loop
+//               iteration that traced the ray (miss, segment extinction
+//               tallies, Russian roulette, point update) and the head of the
+//               next one (extinction, BSDF or virtual pass-through, receiver
+//               tallies); finished realisations flush their pending tallies
+//               with wave-aggregated FP64 atomics and the slot is refilled
+//               with a new realisation in place;
+//   k_scan / k_scatter  order-preserving compaction of the slots that wrote
+//               a ray into the next work list (per-block counts, one-block
+//               scan of the counts, scatter).
 //
-// The host loop alternates k_trace / k_shade until the queue is empty.
+// The host loop runs trace / shade / scan / scatter until the list is empty.
 #include <hip/hip_runtime.h>
 
 #include <float.h>
@@ -28,6 +33,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "ssolh.h"
@@ -38,7 +44,9 @@ using namespace ssolh;
 namespace {
 
 constexpr int BLOCK = 128;           // threads per block (2 waves)
+constexpr int WAVES = BLOCK / 64;
 constexpr int STACK = 48;            // traversal stack entries per thread (LDS)
+constexpr int SCAN_THREADS = 1024;   // the one block of k_scan
 constexpr uint32_t SENTINEL = 0xFFFFFFFFu;
 constexpr uint32_t LEAF = 0x80000000u;
 constexpr uint32_t NONE = 0xFFFFFFFFu;
@@ -67,13 +75,25 @@ void set_err(const char* fmt, ...) {
 // -------------------------------------------------------------------------
 // Device scene
 // -------------------------------------------------------------------------
+// What a hit on (instance, shaded shape) needs, gathered at upload so that a
+// hit costs one lookup: global index = instance.isshape_first + sshape.
+struct alignas(8) IssInfo {
+  uint32_t inst;
+  uint32_t shape;
+  uint32_t mtl_front, mtl_back;
+  int32_t receiver;        // receiver slot of the instance or -1
+  uint32_t receiver_mask;  // ssol_side_flag bits
+  uint32_t shape_type;     // enum ssolh_shape_type
+  uint32_t pad;
+  int64_t prim_tally;      // first per-primitive slot or -1
+};
+
 struct DScene {
   Spectra spectra;
   const ssolh_medium* media;
   uint32_t air;
   const ssolh_material* mtls;
   const ssolh_shape* shapes;
-  const ssolh_sshape* sshapes;
   const ssolh_object* objects;
   const ssolh_bvh_node* nodes;
   uint32_t tlas_root;
@@ -81,6 +101,7 @@ struct DScene {
   const ssolh_instance* insts;
   const uint32_t* tlas_inst;
   const ssolh_inst_sshape* isshapes;
+  const IssInfo* iss;
   const ssolh_sample_tri* stris;
   const ssolh_alias* alias;
   uint32_t nstris;
@@ -89,7 +110,7 @@ struct DScene {
   uint32_t nrcv, nsamp;
 };
 
-// Ray and hit records, indexed by queue position --------------------------
+// Ray and hit records, indexed by slot ------------------------------------
 enum : uint32_t {
   RF_SIDE_SHIFT = 16,     // 2 bits: side of the origin point
   RF_DISCARD_VIRTUAL = 1u << 18,
@@ -98,17 +119,22 @@ enum : uint32_t {
 };
 
 struct alignas(16) RayRec {
-  double ox, oy, oz, tmin;
-  double dx, dy, dz, pad;
-  uint32_t slot, inst_from, prim_from, flags;  // flags: sshape_from | RF_*
+  double ox, oy, oz;
+  double dx, dy, dz;
+  float tmin;
+  uint32_t iss_from;   // (instance, shaded shape) of the emitting point
+  uint32_t prim_from;  // its triangle
+  uint32_t flags;      // side << RF_SIDE_SHIFT | RF_*
 };
+static_assert(sizeof(RayRec) == 64, "RayRec layout");
 
 struct alignas(16) HitRec {
-  double t, qdst;
-  double qnx, qny, qnz;
-  float ngx, ngy, ngz;
-  uint32_t inst, prim, sshape;  // inst == NONE <=> miss
+  double t;             // distance of the nearest accepted triangle
+  double qdst;          // nearest refined quadric distance (punched shapes)
+  double nx, ny, nz;    // unnormalised world normal: triangle Ng, or quadric gradient
+  uint32_t iss, prim;   // iss == NONE <=> miss
 };
+static_assert(sizeof(HitRec) == 48, "HitRec layout");
 
 // Path state, indexed by slot ---------------------------------------------
 enum : uint32_t {
@@ -126,7 +152,7 @@ struct PathState {
   double4* nrm_t;      // normal of the current point, last hit distance
   double4* pend;       // pending absorbed_by_receivers, extinguished, other
   uint4* meta;         // path id lo/hi, rng block, flags
-  uint4* cur;          // inst, prim, sshape | side << 16, sampled slot
+  uint4* cur;          // (instance, shaded shape) index, prim, side, sampled slot
   uint4* misc;         // depth, in medium | out medium << 16, survivor score, roulette interval
   uint32_t* pkey;      // [npend][slots]: receiver key (slot*2+side) or NONE
   double* pval;        // [npend][10][slots]
@@ -135,23 +161,27 @@ struct PathState {
 };
 
 struct Counters {
-  unsigned long long issued;        // realisations handed out
-  unsigned long long realisations;  // completed and counted
   unsigned long long failed;
-  unsigned long long segments;
   unsigned long long spills;
   unsigned int aborted;
-  unsigned int pad;
-  unsigned int queue_count[2];      // ray queue lengths (ping-pong)
+  unsigned int total;               // length of the next work list
 };
 
+// Tallies that every wave updates (global fields, counts) or that few keys
+// share (receivers, sampled instances) are striped: block b adds into
+// stripe b & mask, so concurrent atomics spread over many cache lines; the
+// stripes are folded into stripe 0 after the solve. Keys that are many and
+// scattered (sampled x receiver, per primitive) have one copy.
 struct Tallies {
-  double* global;   // [6][2]
-  double* rcv;      // [nrcv][2][10][2]
-  double* samp;     // [nsamp][2][2]
-  unsigned long long* samp_count;  // [nsamp]
+  double* global;   // [mask_g+1][16]: 6 fields x {w, w2}, one 128-byte line per stripe
+  unsigned long long* realisations;  // [mask_g+1][16]
+  double* rcv;      // [mask_r+1][nrcv][2][10][2]
+  double* samp;     // [mask_s+1][nsamp][2][2]
+  unsigned long long* samp_count;    // [mask_s+1][nsamp]
   double* sxr;      // [nsamp][nrcv][2][10][2]
   double* prim;     // [nprim][2][10][2]
+  uint32_t mask_g, mask_r, mask_s;
+  uint32_t nrcv, nsamp;
 };
 
 struct SolveParams {
@@ -163,8 +193,13 @@ struct SolveParams {
   uint32_t npend;
 };
 
+// Work list: slot of entry q (a null list is the identity)
+__device__ __forceinline__ uint32_t slot_of(const uint32_t* list, uint32_t q) {
+  return list ? list[q] : q;
+}
+
 // -------------------------------------------------------------------------
-// Wave helpers (64-wide)
+// Wave and block helpers (64-wide waves)
 // -------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -209,6 +244,20 @@ __device__ __forceinline__ unsigned long long wave_count_add(bool pred, unsigned
   base = __shfl(base, leader, 64);
   const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
   return base + (unsigned long long)__popcll(below);
+}
+
+// Number of set predicates in the block, written by thread 0 to
+// out[blockIdx.x]. Every thread of the block must call it.
+__device__ void block_count(bool pred, uint32_t* out) {
+  __shared__ uint32_t wcount[WAVES];
+  const uint64_t m = __ballot(pred);
+  if (lane_id() == 0) wcount[threadIdx.x / 64] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int w = 0; w < WAVES; ++w) s += wcount[w];
+    out[blockIdx.x] = s;
+  }
 }
 
 // -------------------------------------------------------------------------
@@ -286,49 +335,48 @@ struct TraceState {
   double qdst;
   d3 qn;
   d3 ng;  // world geometry normal of the best hit (unnormalised)
-  uint32_t inst, prim, sshape;
+  uint32_t iss, prim;
+  bool punched;  // the best triangle belongs to a punched surface
 };
 
 // Returns true if the candidate is accepted (src/ssol_scene.c:438-536).
-__device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t inst_id,
-                       const ssolh_instance& inst, const ssolh_triangle& tri, double t,
-                       TraceState& ts, d3& ng_out) {
-  const uint32_t sshape_from = r.flags & 0xFFFFu;
+__device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t iss_id,
+                       const ssolh_triangle& tri, double t, TraceState& ts, d3& ng_out,
+                       bool& punched) {
   const uint32_t side_from = (r.flags >> RF_SIDE_SHIFT) & 3u;
-  const ssolh_object obj = S.objects[inst.object];
-  const ssolh_sshape ss = S.sshapes[obj.sshape_first + tri.sshape];
-  const ssolh_shape& sh = S.shapes[ss.shape];
+  const IssInfo is = S.iss[iss_id];
   uint32_t side;
   double dst = 0;
   d3 qn = mk(0, 0, 0);
   if (t < r.tmin) return false;
-  if (sh.type == SSOLH_SHAPE_MESH) {
-    if (t <= 1.e-6 || t <= r.tmin ||
-        (tri.prim == r.prim_from && tri.sshape == sshape_from && inst_id == r.inst_from))
+  punched = is.shape_type != SSOLH_SHAPE_MESH;
+  if (!punched) {
+    if (t <= 1.e-6 || t <= r.tmin || (tri.prim == r.prim_from && iss_id == r.iss_from))
       return false;
     const d3 v0 = ld3(tri.v0), v1 = ld3(tri.v1), v2 = ld3(tri.v2);
     const d3 n_obj = cross(v2 - v0, v1 - v0);
-    ng_out = tmul(inst.inv_linear, n_obj);
+    ng_out = tmul(S.insts[is.inst].inv_linear, n_obj);
     side = dot(ng_out, d) < 0 ? SSOL_FRONT_ : SSOL_BACK_;
   } else {
-    const ssolh_inst_sshape& is = S.isshapes[inst.isshape_first + tri.sshape];
-    const d3 lo = mul(is.R_inv, o - ld3(is.T));
-    const d3 ldir = mul(is.R_inv, d);
+    const ssolh_inst_sshape& q = S.isshapes[iss_id];
+    const ssolh_shape& sh = S.shapes[is.shape];
+    const d3 lo = mul(q.R_inv, o - ld3(q.T));
+    const d3 ldir = mul(q.R_inv, d);
     d3 grad;
     if (!quadric_hit(sh.quadric, sh.qparam, lo, ldir, t, dst, grad)) return false;
     if (!(dst < (double)FLT_MAX)) return false;
     if ((double)(float)dst <= r.tmin) return false;
-    qn = normalize(tmul(is.R_inv, grad));
+    qn = normalize(tmul(q.R_inv, grad));
     side = dot(d, qn) < 0 ? SSOL_FRONT_ : SSOL_BACK_;
-    if (inst_id == r.inst_from && tri.sshape == sshape_from && side != side_from) return false;
+    if (iss_id == r.iss_from && side != side_from) return false;
   }
   if (r.flags & RF_REVERSED) side = side == SSOL_FRONT_ ? SSOL_BACK_ : SSOL_FRONT_;
-  const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? ss.mtl_front : ss.mtl_back];
+  const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? is.mtl_front : is.mtl_back];
   if (m.type == SSOLH_MTL_VIRTUAL) {
     if (r.flags & RF_DISCARD_VIRTUAL) return false;
-    if (!(inst.receiver_mask & side)) return false;
+    if (!(is.receiver_mask & side)) return false;
   }
-  if (sh.type != SSOLH_SHAPE_MESH && ts.qdst >= dst) {
+  if (punched && ts.qdst >= dst) {
     ts.qdst = dst;
     ts.qn = qn;
   }
@@ -336,20 +384,14 @@ __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t in
 }
 
 __global__ void __launch_bounds__(BLOCK)
-k_trace(DScene S, const RayRec* __restrict__ rays, HitRec* __restrict__ hits,
-        const unsigned int* __restrict__ count_ptr, unsigned long long* segments) {
+k_trace(DScene S, const uint32_t* __restrict__ list, uint32_t count,
+        const RayRec* __restrict__ rays, HitRec* __restrict__ hits) {
   __shared__ uint32_t lds_stack[STACK * BLOCK];
   const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-  const uint32_t count = *count_ptr;
-  const bool live = q < count;
-  // one segment per ray, counted per wave
-  {
-    const uint64_t m = __ballot(live);
-    if (lane_id() == 0 && m) atomicAdd(segments, (unsigned long long)__popcll(m));
-  }
-  if (!live) return;
+  if (q >= count) return;
 
-  const RayRec r = rays[q];
+  const uint32_t slot = slot_of(list, q);
+  const RayRec r = rays[slot];
   const d3 ow = mk(r.ox, r.oy, r.oz), dw = mk(r.dx, r.dy, r.dz);
   const bool any_hit = (r.flags & RF_ANY_HIT) != 0;
   TraceState ts;
@@ -357,41 +399,41 @@ k_trace(DScene S, const RayRec* __restrict__ rays, HitRec* __restrict__ hits,
   ts.qdst = INFINITY;
   ts.qn = mk(0, 0, 0);
   ts.ng = mk(0, 0, 0);
-  ts.inst = NONE;
+  ts.iss = NONE;
   ts.prim = 0;
-  ts.sshape = 0;
+  ts.punched = false;
 
   uint32_t* stack = lds_stack + threadIdx.x;
   int sp = 0;
-  const FRay fw = make_fray(ow, dw);
-  FRay fr = fw;
+  FRay fr = make_fray(ow, dw);
   d3 o = ow, d = dw;        // current-level ray (double)
   Shear sh = make_shear(dw);
-  uint32_t cur_inst = NONE;  // instance whose BLAS is being traversed
-  ssolh_instance inst{};
-  const float ftmin = (float)r.tmin;
+  bool in_blas = false;
+  uint32_t iss_first = 0;   // (instance, sshape) base of the BLAS being traversed
+  const float ftmin = r.tmin;
   bool done = false;
 
   uint32_t ref = S.tlas_root;  // inner node reference
   for (;;) {
     if (ref == SENTINEL) {
       // back to the TLAS
-      cur_inst = NONE;
-      fr = fw;
+      in_blas = false;
+      fr = make_fray(ow, dw);
       o = ow;
       d = dw;
     } else if (ref & LEAF) {
       const uint32_t first = ref & 0x0FFFFFFFu;
       const uint32_t cnt = ((ref >> 28) & 7u) + 1u;
-      if (cur_inst == NONE) {
+      if (!in_blas) {
         // TLAS leaf: one instance, enter its BLAS
-        cur_inst = S.tlas_inst[first];
-        inst = S.insts[cur_inst];
+        const ssolh_instance& inst = S.insts[S.tlas_inst[first]];
         const d3 T = mk(inst.transform[9], inst.transform[10], inst.transform[11]);
         o = mul(inst.inv_linear, ow - T);
         d = mul(inst.inv_linear, dw);
         fr = make_fray(o, d);
         sh = make_shear(d);
+        iss_first = inst.isshape_first;
+        in_blas = true;
         if (sp < STACK) { stack[sp * BLOCK] = SENTINEL; ++sp; }
         ref = S.objects[inst.object].blas_root;
         continue;
@@ -401,12 +443,14 @@ k_trace(DScene S, const RayRec* __restrict__ rays, HitRec* __restrict__ hits,
         const double t = ray_triangle(o, sh, ld3(tri.v0), ld3(tri.v1), ld3(tri.v2));
         if (!(t >= r.tmin && t < ts.tbest)) continue;
         d3 ng;
-        if (!filter(S, r, ow, dw, cur_inst, inst, tri, t, ts, ng)) continue;
+        bool punched;
+        const uint32_t iss = iss_first + tri.sshape;
+        if (!filter(S, r, ow, dw, iss, tri, t, ts, ng, punched)) continue;
         ts.tbest = t;
-        ts.inst = cur_inst;
+        ts.iss = iss;
         ts.prim = tri.prim;
-        ts.sshape = tri.sshape;
         ts.ng = ng;
+        ts.punched = punched;
         if (any_hit) { done = true; break; }
       }
       if (done) break;
@@ -441,12 +485,11 @@ k_trace(DScene S, const RayRec* __restrict__ rays, HitRec* __restrict__ hits,
   HitRec h;
   h.t = ts.tbest;
   h.qdst = ts.qdst;
-  h.qnx = ts.qn.x; h.qny = ts.qn.y; h.qnz = ts.qn.z;
-  h.ngx = (float)ts.ng.x; h.ngy = (float)ts.ng.y; h.ngz = (float)ts.ng.z;
-  h.inst = ts.inst;
+  const d3 n = ts.punched ? ts.qn : ts.ng;
+  h.nx = n.x; h.ny = n.y; h.nz = n.z;
+  h.iss = ts.iss;
   h.prim = ts.prim;
-  h.sshape = ts.sshape;
-  hits[q] = h;
+  hits[slot] = h;
 }
 
 // -------------------------------------------------------------------------
@@ -475,7 +518,7 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   const ssolh_alias al = S.alias[i];
   const uint32_t idx = (x - (double)i) < al.prob ? i : al.alias;
   const ssolh_sample_tri st = S.stris[idx];
-  const ssolh_instance inst = S.insts[st.instance];
+  const ssolh_instance& inst = S.insts[st.instance];
   const double sq = sqrt(u1);
   const double b0 = 1 - sq, b1 = sq * (1 - u2), b2 = sq * u2;
   const d3 v0 = ld3(st.v0), v1 = ld3(st.v1), v2 = ld3(st.v2);
@@ -486,17 +529,18 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   const d3 dir = sample_sun(S.sun, rng);
   const double wl = sample_wavelength(S.spectra, S.sun.wl_spectrum, rng.next());
 
-  const ssolh_object obj = S.objects[inst.object];
-  const ssolh_sshape ss = S.sshapes[obj.sshape_first + st.sshape];
-  const ssolh_shape& sh = S.shapes[ss.shape];
+  const uint32_t iss = inst.isshape_first + st.sshape;
+  const IssInfo is = S.iss[iss];
   d3 N = ns;
-  if (sh.type == SSOLH_SHAPE_PUNCHED) {
+  const bool punched = is.shape_type == SSOLH_SHAPE_PUNCHED;
+  if (punched) {
     // lift onto the quadric (src/ssol_shape.c:1408-1447)
-    const ssolh_inst_sshape& is = S.isshapes[inst.isshape_first + st.sshape];
-    d3 loc = mul(is.R_inv, pos - ld3(is.T));
+    const ssolh_shape& sh = S.shapes[is.shape];
+    const ssolh_inst_sshape& q = S.isshapes[iss];
+    d3 loc = mul(q.R_inv, pos - ld3(q.T));
     loc.z = quadric_z(sh.quadric, sh.qparam, loc.x, loc.y);
-    N = normalize(tmul(is.R_inv, quadric_grad(sh.quadric, sh.qparam, loc)));
-    pos = mul(is.R, loc) + ld3(is.T);
+    N = normalize(tmul(q.R_inv, quadric_grad(sh.quadric, sh.qparam, loc)));
+    pos = mul(q.R, loc) + ld3(q.T);
   }
   uint32_t side = SSOL_FRONT_;
   if (dot(N, dir) >= 0) { side = SSOL_BACK_; N = -N; }
@@ -505,11 +549,11 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   const double surface_sun_cos = dot(N, dir);
   const double surface_sun0_cos = fabs(dot(sun0, N));
   const double sun0_sun_cos = dot(sun0, dir);
-  const double proxy_cos = sh.type == SSOLH_SHAPE_MESH ? 1.0 : fabs(dot(ns, N));
+  const double proxy_cos = punched ? fabs(dot(ns, N)) : 1.0;
   const double w0 = S.sun.dni * S.S_proxy * fabs(surface_sun_cos / (proxy_cos * sun0_sun_cos));
   const double cosf = S.S_proxy / S.S * surface_sun0_cos / proxy_cos;
 
-  const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? ss.mtl_front : ss.mtl_back];
+  const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? is.mtl_front : is.mtl_back];
   const uint32_t med = start_medium(S, m, side);
 
   P.pos_wl[slot] = make_double4(pos.x, pos.y, pos.z, wl);
@@ -519,7 +563,7 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   P.nrm_t[slot] = make_double4(N.x, N.y, N.z, 0);
   P.pend[slot] = make_double4(0, 0, 0, 0);
   P.meta[slot] = make_uint4((uint32_t)path, (uint32_t)(path >> 32), rng.block, PF_ACTIVE);
-  P.cur[slot] = make_uint4(st.instance, st.prim, st.sshape | (side << 16), (uint32_t)inst.sampled);
+  P.cur[slot] = make_uint4(iss, st.prim, side, (uint32_t)inst.sampled);
   P.misc[slot] = make_uint4(0, med | (med << 16), 0, 64);
   for (uint32_t k = 0; k < prm.npend; ++k) {
     P.pkey[(size_t)k * prm.slots + slot] = NONE;
@@ -529,58 +573,40 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
 
   // shadow ray toward the sun (src/ssol_solver.c:325-352)
   e.has = true;
-  e.r.ox = pos.x; e.r.oy = pos.y; e.r.oz = pos.z; e.r.tmin = 0;
-  e.r.dx = -dir.x; e.r.dy = -dir.y; e.r.dz = -dir.z; e.r.pad = 0;
-  e.r.slot = slot;
-  e.r.inst_from = st.instance;
+  e.r.ox = pos.x; e.r.oy = pos.y; e.r.oz = pos.z;
+  e.r.dx = -dir.x; e.r.dy = -dir.y; e.r.dz = -dir.z;
+  e.r.tmin = 0;
+  e.r.iss_from = iss;
   e.r.prim_from = st.prim;
-  e.r.flags = st.sshape | (side << RF_SIDE_SHIFT) | RF_DISCARD_VIRTUAL | RF_REVERSED | RF_ANY_HIT;
-}
-
-// Append the live rays of the wave to the next queue, one atomic per wave
-__device__ void emit_rays(const Emit& e, RayRec* out, unsigned int* count) {
-  const uint64_t mask = __ballot(e.has);
-  if (!mask) return;
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  unsigned int base = 0;
-  if ((int)lane_id() == leader) base = atomicAdd(count, (unsigned int)__popcll(mask));
-  base = __shfl(base, leader, 64);
-  if (e.has) {
-    const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
-    out[base + (unsigned)__popcll(below)] = e.r;
-  }
+  e.r.flags = (side << RF_SIDE_SHIFT) | RF_DISCARD_VIRTUAL | RF_REVERSED | RF_ANY_HIT;
 }
 
 __device__ void flush_sampled_count(bool has, int samp_slot, const Tallies& T) {
   const uint64_t key = has ? (uint64_t)samp_slot : 0;
   const uint64_t mask = __ballot(has);
   if (!mask) return;
+  unsigned long long* base = T.samp_count + (size_t)(blockIdx.x & T.mask_s) * T.nsamp;
   const int leader = __ffsll((unsigned long long)mask) - 1;
   const uint64_t lkey = __shfl(key, leader, 64);
   const bool match = has && key == lkey;
   const uint64_t mm = __ballot(match);
-  if ((int)lane_id() == leader) atomicAdd(T.samp_count + lkey, (unsigned long long)__popcll(mm));
-  if (has && !match) atomicAdd(T.samp_count + key, 1ull);
+  if ((int)lane_id() == leader) atomicAdd(base + lkey, (unsigned long long)__popcll(mm));
+  if (has && !match) atomicAdd(base + key, 1ull);
 }
 
+// First realisations: slot s holds realisation s, s < count. Slot s then
+// runs realisations s + k * slots, k = 1, 2, ... while they are below N.
 __global__ void __launch_bounds__(BLOCK)
-k_generate(DScene S, PathState P, SolveParams prm, Tallies T, RayRec* out,
-           unsigned int* out_count, Counters* C) {
+k_generate(DScene S, PathState P, SolveParams prm, Tallies T, RayRec* __restrict__ rays,
+           uint32_t count) {
   const uint32_t slot = blockIdx.x * BLOCK + threadIdx.x;
+  const bool live = slot < count;
   Emit e;
   e.has = false;
   int samp_slot = 0;
-  bool gen = false;
-  if (slot < prm.slots) {
-    P.meta[slot] = make_uint4(0, 0, 0, 0);
-  }
-  const unsigned long long id = wave_count_add(slot < prm.slots, &C->issued);
-  if (slot < prm.slots && id < prm.N) {
-    generate(S, P, prm, slot, id, e, samp_slot);
-    gen = true;
-  }
-  flush_sampled_count(gen, samp_slot, T);
-  emit_rays(e, out, out_count);
+  if (live) generate(S, P, prm, slot, slot, e, samp_slot);
+  flush_sampled_count(live, samp_slot, T);
+  if (e.has) rays[slot] = e.r;
 }
 
 // -------------------------------------------------------------------------
@@ -625,11 +651,10 @@ __device__ void spill(double* base, uint64_t key, const double* w, unsigned long
 }
 
 __global__ void __launch_bounds__(BLOCK)
-k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restrict__ rays,
-        const HitRec* __restrict__ hits, const unsigned int* __restrict__ count_ptr,
-        RayRec* __restrict__ out, unsigned int* out_count, Counters* C) {
+k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __restrict__ list,
+        uint32_t count, RayRec* __restrict__ rays, const HitRec* __restrict__ hits,
+        uint8_t* __restrict__ emitted, uint32_t* __restrict__ block_emits, Counters* C) {
   const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-  const uint32_t count = *count_ptr;
   const bool live = q < count;
   const size_t stride = prm.slots;
 
@@ -640,12 +665,11 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restr
   fin.factor = 1;
   fin.samp_slot = 0;
   for (int i = 0; i < 6; ++i) fin.g[i] = 0;
-  uint32_t slot = 0;
+  const uint32_t slot = live ? slot_of(list, q) : 0;
 
   if (live) {
-    const RayRec r = rays[q];
-    const HitRec h = hits[q];
-    slot = r.slot;
+    const RayRec r = rays[slot];
+    const HitRec h = hits[slot];
     uint4 meta = P.meta[slot];
     uint4 cur = P.cur[slot];
     uint4 misc = P.misc[slot];
@@ -670,11 +694,11 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restr
     double hit_t = 0;
     d3 rorg = pos, rdir = dir;  // origin/direction of the ray of the point
     bool head = false;
-    uint32_t inst_id = cur.x, prim = cur.y, sshape = cur.z & 0xFFFFu, side = cur.z >> 16;
+    uint32_t iss = cur.x, prim = cur.y, side = cur.z;
 
     if (!(flags & PF_STAGE_BOUNCE)) {
       // result of the shadow ray
-      if (h.inst != NONE) {
+      if (h.iss != NONE) {
         fin.kind = FIN_DONE;
         fin.g[0] = dc.w;      // cos factor
         fin.g[2] = pv.w;      // shadowed = initial flux
@@ -684,7 +708,7 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restr
       }
     } else {
       // tail of the iteration that traced this ray
-      const bool found = h.inst != NONE;
+      const bool found = h.iss != NONE;
       const bool hit_virtual = (flags & PF_HIT_VIRTUAL) != 0;
       const bool in_atm = (flags & PF_IN_ATM) != 0;
       bool last = false;
@@ -717,18 +741,11 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restr
         // point_update_from_hit (src/ssol_solver.c:360-411)
         rorg = mk(r.ox, r.oy, r.oz);
         rdir = mk(r.dx, r.dy, r.dz);
-        inst_id = h.inst;
+        iss = h.iss;
         prim = h.prim;
-        sshape = h.sshape;
-        const ssolh_instance inst = S.insts[inst_id];
-        const ssolh_sshape ss = S.sshapes[S.objects[inst.object].sshape_first + sshape];
-        if (S.shapes[ss.shape].type == SSOLH_SHAPE_MESH) {
-          N = normalize(mk(h.ngx, h.ngy, h.ngz));
-          pos = rorg + rdir * h.t;
-        } else {
-          N = normalize(mk(h.qnx, h.qny, h.qnz));
-          pos = rorg + dir * h.qdst;
-        }
+        N = normalize(mk(h.nx, h.ny, h.nz));
+        if (S.iss[iss].shape_type == SSOLH_SHAPE_MESH) pos = rorg + rdir * h.t;
+        else pos = rorg + dir * h.qdst;
         if (dot(dir, N) < 0) side = SSOL_FRONT_;
         else { side = SSOL_BACK_; N = -N; }
         in_med = out_med;
@@ -739,11 +756,10 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restr
 
     if (head) {
       if (!(flags & PF_STAGE_BOUNCE)) { rorg = pos; rdir = dir; }
-      const ssolh_instance inst = S.insts[inst_id];
-      const ssolh_sshape ss = S.sshapes[S.objects[inst.object].sshape_first + sshape];
-      const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? ss.mtl_front : ss.mtl_back];
+      const IssInfo is = S.iss[iss];
+      const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? is.mtl_front : is.mtl_back];
       const bool in_atm = in_med == S.air;
-      const bool receiver = (inst.receiver_mask & side) != 0;
+      const bool receiver = (is.receiver_mask & side) != 0;
       const bool virt = m.type == SSOLH_MTL_VIRTUAL;
       double trans = 1;
       if (hit_t > 0) {
@@ -793,12 +809,11 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restr
           w10[3] = in2 - in0; w10[4] = in1 - in0;
           w10[5] = in0 * kabs; w10[6] = in1 * kabs; w10[7] = in2 * kabs;
           w10[8] = (in2 - in0) * kabs; w10[9] = (in1 - in0) * kabs;
-          const uint32_t rkey = (uint32_t)inst.receiver * 2u + (side == SSOL_FRONT_ ? 0u : 1u);
+          const uint32_t rkey = (uint32_t)is.receiver * 2u + (side == SSOL_FRONT_ ? 0u : 1u);
           if (!pend_add(P.pkey + slot, P.pval + slot, stride, prm.npend, rkey, w10)) {
-            spill(T.rcv, rkey, w10, &C->spills);
+            spill(T.rcv + (size_t)(blockIdx.x & T.mask_r) * T.nrcv * 40u, rkey, w10, &C->spills);
             spill(T.sxr, (uint64_t)cur.w * S.nrcv * 2u + rkey, w10, &C->spills);
           }
-          const ssolh_inst_sshape& is = S.isshapes[inst.isshape_first + sshape];
           if (is.prim_tally >= 0) {
             const uint32_t pkey = (uint32_t)(((uint64_t)is.prim_tally + prim) * 2u +
                                              (side == SSOL_FRONT_ ? 0u : 1u));
@@ -819,17 +834,15 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restr
           if (virt) {
             e.r.ox = rorg.x; e.r.oy = rorg.y; e.r.oz = rorg.z;
             e.r.dx = rdir.x; e.r.dy = rdir.y; e.r.dz = rdir.z;
-            e.r.tmin = (double)nextafterf((float)hit_t, FLT_MAX);
+            e.r.tmin = nextafterf((float)hit_t, FLT_MAX);
           } else {
             e.r.ox = pos.x; e.r.oy = pos.y; e.r.oz = pos.z;
             e.r.dx = dir.x; e.r.dy = dir.y; e.r.dz = dir.z;
             e.r.tmin = 0;
           }
-          e.r.pad = 0;
-          e.r.slot = slot;
-          e.r.inst_from = inst_id;
+          e.r.iss_from = iss;
           e.r.prim_from = prim;
-          e.r.flags = sshape | (side << RF_SIDE_SHIFT);
+          e.r.flags = side << RF_SIDE_SHIFT;
           flags = PF_ACTIVE | PF_STAGE_BOUNCE | (virt ? PF_HIT_VIRTUAL : 0u) | (in_atm ? PF_IN_ATM : 0u);
           oi = make_double4(out0, out1, out2, in0);
           out_med = next_med;
@@ -853,23 +866,27 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restr
       P.nrm_t[slot] = make_double4(N.x, N.y, N.z, hit_t);
       P.pend[slot] = pd;
       P.meta[slot] = make_uint4(meta.x, meta.y, rng.block, flags);
-      P.cur[slot] = make_uint4(inst_id, prim, sshape | (side << 16), cur.w);
+      P.cur[slot] = make_uint4(iss, prim, side, cur.w);
       P.misc[slot] = make_uint4(depth, in_med | (out_med << 16), score, rinterval);
     }
   }
 
   // --- convergent flush of finished realisations -------------------------
   const bool done = fin.kind == FIN_DONE;
+  double* rcv_base = T.rcv + (size_t)(blockIdx.x & T.mask_r) * T.nrcv * 40u;
   {
+    const uint32_t sg = blockIdx.x & T.mask_g;
     double g2[12];
     for (int i = 0; i < 6; ++i) {
       g2[2 * i] = done ? fin.g[i] : 0.0;
       g2[2 * i + 1] = done ? fin.g[i] * fin.g[i] : 0.0;
     }
-    keyed_add<12>(done, 0, g2, T.global);
+    keyed_add<12>(done, 0, g2, T.global + (size_t)sg * 16u);
     double s4[4] = {fin.g[0], fin.g[0] * fin.g[0], fin.g[2], fin.g[2] * fin.g[2]};
-    keyed_add<4>(done, (uint64_t)fin.samp_slot, s4, T.samp);
-    wave_count_add(done || fin.kind == FIN_KILLED, &C->realisations);
+    keyed_add<4>(done, (uint64_t)fin.samp_slot, s4,
+                 T.samp + (size_t)(blockIdx.x & T.mask_s) * T.nsamp * 4u);
+    const uint64_t m = __ballot(done || fin.kind == FIN_KILLED);
+    if (lane_id() == 0 && m) atomicAdd(T.realisations + (size_t)sg * 16u, (unsigned long long)__popcll(m));
     const unsigned long long nf = wave_count_add(fin.kind == FIN_FAILED, &C->failed);
     if (fin.kind == FIN_FAILED && nf + 1 >= prm.max_failed) atomicExch(&C->aborted, 1u);
   }
@@ -889,7 +906,7 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restr
     }
     const bool has = done && key != NONE;
     if (!has) for (int f = 0; f < 20; ++f) v[f] = 0;
-    keyed_add<20>(has, has ? key : 0, v, T.rcv);
+    keyed_add<20>(has, has ? key : 0, v, rcv_base);
     const uint64_t xkey = has ? (uint64_t)fin.samp_slot * (uint64_t)S.nrcv * 2u + key : 0;
     keyed_add<20>(has, xkey, v, T.sxr);
   }
@@ -913,9 +930,9 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restr
     }
   }
 
-  // --- refill finished slots with new realisations ------------------------
+  // --- refill finished slots with their next realisation -------------------
   const bool want = live && fin.kind != FIN_NONE && !C->aborted;
-  const unsigned long long id = wave_count_add(want, &C->issued);
+  const uint64_t id = want ? (((uint64_t)P.meta[slot].y << 32) | P.meta[slot].x) + prm.slots : 0;
   int samp_slot = 0;
   bool gen = false;
   if (want && id < prm.N) {
@@ -925,7 +942,71 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const RayRec* __restr
     P.meta[slot] = make_uint4(0, 0, 0, 0);
   }
   flush_sampled_count(gen, samp_slot, T);
-  emit_rays(e, out, out_count);
+
+  // --- the slot's next ray and its place in the next work list -----------
+  if (e.has) rays[slot] = e.r;
+  if (live) emitted[slot] = e.has ? 1 : 0;
+  block_count(e.has, block_emits);
+}
+
+// -------------------------------------------------------------------------
+// Order-preserving compaction of the work list
+// -------------------------------------------------------------------------
+// Exclusive scan of the per-block emit counts of k_shade, by one block;
+// the grand total is the next list's length.
+__global__ void __launch_bounds__(SCAN_THREADS)
+k_scan(const uint32_t* __restrict__ counts, uint32_t nblocks, uint32_t* __restrict__ offsets,
+       unsigned int* total) {
+  __shared__ uint32_t part[SCAN_THREADS];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t per = (nblocks + SCAN_THREADS - 1) / SCAN_THREADS;
+  const uint32_t b0 = min(tid * per, nblocks), b1 = min(b0 + per, nblocks);
+  uint32_t s = 0;
+  for (uint32_t b = b0; b < b1; ++b) s += counts[b];
+  part[tid] = s;
+  __syncthreads();
+  for (uint32_t off = 1; off < SCAN_THREADS; off <<= 1) {
+    const uint32_t v = tid >= off ? part[tid - off] : 0u;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = tid ? part[tid - 1] : 0u;
+  for (uint32_t b = b0; b < b1; ++b) {
+    offsets[b] = run;
+    run += counts[b];
+  }
+  if (tid == SCAN_THREADS - 1) *total = part[SCAN_THREADS - 1];
+}
+
+// Entry q of the next list: the q-th slot (in list order) that wrote a ray.
+// Same block partition as k_shade.
+__global__ void __launch_bounds__(BLOCK)
+k_scatter(const uint32_t* __restrict__ list, uint32_t count, const uint8_t* __restrict__ emitted,
+          const uint32_t* __restrict__ offsets, uint32_t* __restrict__ next) {
+  __shared__ uint32_t wcount[WAVES];
+  const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
+  const bool live = q < count;
+  const uint32_t slot = live ? slot_of(list, q) : 0;
+  const bool pred = live && emitted[slot];
+  const uint64_t m = __ballot(pred);
+  const uint32_t w = threadIdx.x / 64;
+  if (lane_id() == 0) wcount[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t base = offsets[blockIdx.x];
+  for (uint32_t i = 0; i < w; ++i) base += wcount[i];
+  const uint64_t below = m & ((1ull << lane_id()) - 1ull);
+  if (pred) next[base + (uint32_t)__popcll(below)] = slot;
+}
+
+// Sum of the stripes of a striped tally into stripe 0
+template <typename T>
+__global__ void k_fold(T* base, size_t n, uint32_t stripes) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T s = base[i];
+  for (uint32_t k = 1; k < stripes; ++k) s += base[(size_t)k * n + i];
+  base[i] = s;
 }
 
 // -------------------------------------------------------------------------
@@ -947,34 +1028,34 @@ __global__ void k_test_quadric(int quadric, double4 qp, uint32_t n, const double
   }
 }
 
+// Uniforms from an array (0.5 once it is exhausted), for fixed-input tests
 struct ArrayRng {
   const double* u;
-  int n, i;
+  uint32_t n, i;
+  __device__ double next() { return i < n ? u[i++] : 0.5; }
 };
 
-__global__ void k_test_sun(ssolh_sun sun, uint32_t n, const double* u, double* out) {
+__global__ void k_test_sun(ssolh_sun sun, uint32_t n, const double* u, uint32_t nu,
+                           double* out, uint32_t* used) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  // pillbox / gaussian consume exactly two uniforms: run them through the
-  // same formulas as sample_sun with the given values
-  const double u0 = u[2 * i], u1 = u[2 * i + 1];
-  double st, ct, phi;
-  d3 loc, d;
-  if (sun.type == SSOLH_SUN_PILLBOX) {
-    const double s2 = u0 * sun.param[0];
-    st = sqrt(s2); ct = sqrt(1 - s2); phi = u1 * 2 * PI;
-    loc = mk(cos(phi) * st, sin(phi) * st, ct);
-    d = normalize(mul(sun.basis, loc));
-  } else if (sun.type == SSOLH_SUN_GAUSSIAN) {
-    st = sun.param[0] * sqrt(-2 * log(u0));
-    st = st > 1 ? 1 : st;
-    ct = sqrt(1 - st * st); phi = u1 * 2 * PI;
-    loc = mk(cos(phi) * st, sin(phi) * st, ct);
-    d = normalize(mul(sun.basis, loc));
-  } else {
-    d = ld3(sun.dir);
-  }
+  ArrayRng rng{u + (size_t)i * nu, nu, 0};
+  const d3 d = sample_sun(sun, rng);
   out[3 * i] = d.x; out[3 * i + 1] = d.y; out[3 * i + 2] = d.z;
+  used[i] = rng.i;
+}
+
+__global__ void k_test_bsdf(ssolh_material m, double4 eta, uint32_t n, const double* wi,
+                            const double* nrm, const double* u, uint32_t nu, double* wo,
+                            double* w, int* lobe) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ArrayRng rng{u + (size_t)i * nu, nu, 0};
+  d3 o;
+  int l;
+  w[i] = bsdf_sample(m, eta.x, eta.y, eta.z, eta.w, ld3(wi + 3 * i), ld3(nrm + 3 * i), rng, o, l);
+  wo[3 * i] = o.x; wo[3 * i + 1] = o.y; wo[3 * i + 2] = o.z;
+  lobe[i] = l;
 }
 
 // Plain device-to-device copy: reads and writes exactly 16 B per element.
@@ -995,9 +1076,9 @@ __global__ void k_test_threefry(uint32_t n, const uint64_t* ctr, const uint64_t*
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
+// -------------------------------------------------------------------------
 // Host side
-// ---------------------------------------------------------------------------
+// -------------------------------------------------------------------------
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
@@ -1014,13 +1095,14 @@ struct ssolh_context {
   uint32_t nrcv = 0, nsamp = 0;
   uint64_t nprim = 0;
   // solve buffers
-  DevBuf state, rays[2], hits, counters, tallies;
+  DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, counters, tallies;
   PathState P{};
   Tallies T{};
-  size_t state_slots = 0, state_npend = 0;
   size_t tally_bytes = 0;
   // last solve
   Counters last{};
+  unsigned long long realisations = 0;
+  uint64_t segments = 0;
   double kernel_ms = 0, trace_ms = 0, shade_ms = 0;
   uint32_t iterations = 0;
 };
@@ -1052,6 +1134,34 @@ void free_scene(ssolh_context* c) {
   for (auto& b : c->scene_bufs) if (b.p) (void)hipFree(b.p);
   c->scene_bufs.clear();
   c->has_scene = false;
+}
+
+// Per (instance, shaded shape) records of the hit path, from the flat scene
+int build_iss(const ssolh_scene* s, std::vector<IssInfo>& out) {
+  out.assign(s->ninst_sshapes, IssInfo{});
+  for (uint32_t i = 0; i < s->ninstances; ++i) {
+    const ssolh_instance& in = s->instances[i];
+    if (in.object >= s->nobjects) return -1;
+    const ssolh_object& o = s->objects[in.object];
+    for (uint32_t k = 0; k < o.sshape_count; ++k) {
+      const uint64_t g = (uint64_t)in.isshape_first + k;
+      const uint32_t ss = o.sshape_first + k;
+      if (g >= s->ninst_sshapes || ss >= s->nsshapes) return -1;
+      const ssolh_sshape& sh = s->sshapes[ss];
+      if (sh.shape >= s->nshapes || sh.mtl_front >= s->nmaterials || sh.mtl_back >= s->nmaterials)
+        return -1;
+      IssInfo& d = out[g];
+      d.inst = i;
+      d.shape = sh.shape;
+      d.mtl_front = sh.mtl_front;
+      d.mtl_back = sh.mtl_back;
+      d.receiver = in.receiver;
+      d.receiver_mask = in.receiver_mask;
+      d.shape_type = (uint32_t)s->shapes[sh.shape].type;
+      d.prim_tally = s->inst_sshapes[g].prim_tally;
+    }
+  }
+  return 0;
 }
 
 }  // namespace
@@ -1095,7 +1205,8 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_scene(c);
-  for (DevBuf* b : {&c->state, &c->rays[0], &c->rays[1], &c->hits, &c->counters, &c->tallies})
+  for (DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
+                    &c->block_emits, &c->block_offsets, &c->counters, &c->tallies})
     if (b->p) (void)hipFree(b->p);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1108,6 +1219,11 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   if (!s->nsample_tris || !s->ninstances || !s->nnodes) {
     set_err("ssolh_scene_upload: empty scene (sampled triangles %u, instances %u)",
             s->nsample_tris, s->ninstances);
+    return SSOLH_ERR_ARG;
+  }
+  std::vector<IssInfo> iss;
+  if (build_iss(s, iss)) {
+    set_err("ssolh_scene_upload: inconsistent instance / shaded-shape indices");
     return SSOLH_ERR_ARG;
   }
   HIP_TRY(hipSetDevice(c->device));
@@ -1124,13 +1240,13 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   r |= upload(c, s->media, s->nmedia, &d.media);
   r |= upload(c, s->materials, s->nmaterials, &d.mtls);
   r |= upload(c, s->shapes, s->nshapes, &d.shapes);
-  r |= upload(c, s->sshapes, s->nsshapes, &d.sshapes);
   r |= upload(c, s->objects, s->nobjects, &d.objects);
   r |= upload(c, s->nodes, s->nnodes, &d.nodes);
   r |= upload(c, s->triangles, s->ntriangles, &d.tris);
   r |= upload(c, s->instances, s->ninstances, &d.insts);
   r |= upload(c, s->tlas_instances, s->ninstances, &d.tlas_inst);
   r |= upload(c, s->inst_sshapes, s->ninst_sshapes, &d.isshapes);
+  r |= upload(c, iss.data(), iss.size(), &d.iss);
   r |= upload(c, s->sample_tris, s->nsample_tris, &d.stris);
   r |= upload(c, s->sample_alias, s->nsample_tris, &d.alias);
   if (r) return r;
@@ -1145,7 +1261,7 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   c->nrcv = s->nreceivers;
   c->nsamp = s->nsampled;
   c->nprim = s->nprim_tallies;
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));  // the host arrays may go after return
   c->has_scene = true;
   return 0;
 }
@@ -1163,67 +1279,76 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
   uint64_t slots64 = a->path_slots ? a->path_slots : (1u << 21);
   if (slots64 > a->realisations) slots64 = a->realisations;
   slots64 = (slots64 + BLOCK - 1) / BLOCK * BLOCK;
+  if (slots64 > (1ull << 28)) { set_err("ssolh_solve: at most 2^28 path slots"); return SSOLH_ERR_LIMIT; }
   const uint32_t slots = (uint32_t)slots64;
+  const uint32_t max_blocks = slots / BLOCK;
 
-  // path state
+  // path state: 9 arrays of 16/32-byte records and the pending entries,
+  // each array starting on a 256-byte boundary
   const size_t n = slots;
-  const size_t bytes_state = n * (6 * sizeof(double4) + 3 * sizeof(uint4)) +
-                             (size_t)npend * n * (2 * sizeof(uint32_t) + 20 * sizeof(double));
+  const size_t sizes[13] = {n * sizeof(double4), n * sizeof(double4), n * sizeof(double4),
+                            n * sizeof(double4), n * sizeof(double4), n * sizeof(double4),
+                            n * sizeof(uint4), n * sizeof(uint4), n * sizeof(uint4),
+                            npend * n * sizeof(uint32_t), npend * n * sizeof(uint32_t),
+                            npend * n * 10 * sizeof(double), npend * n * 10 * sizeof(double)};
+  size_t bytes_state = 0;
+  for (size_t sz : sizes) bytes_state += (sz + 255) / 256 * 256;
   int r = dev_alloc(c->state, bytes_state);
   if (r) return r;
-  char* p = (char*)c->state.p;
-  auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) / 256 * 256; return q; };
-  // (256-byte rounding fits inside the allocation: counted below)
-  c->P.pos_wl = (double4*)take(n * sizeof(double4));
-  c->P.dir_cos = (double4*)take(n * sizeof(double4));
-  c->P.prev_init = (double4*)take(n * sizeof(double4));
-  c->P.out_in0 = (double4*)take(n * sizeof(double4));
-  c->P.nrm_t = (double4*)take(n * sizeof(double4));
-  c->P.pend = (double4*)take(n * sizeof(double4));
-  c->P.meta = (uint4*)take(n * sizeof(uint4));
-  c->P.cur = (uint4*)take(n * sizeof(uint4));
-  c->P.misc = (uint4*)take(n * sizeof(uint4));
-  c->P.pkey = (uint32_t*)take(npend * n * sizeof(uint32_t));
-  c->P.qkey = (uint32_t*)take(npend * n * sizeof(uint32_t));
-  c->P.pval = (double*)take(npend * n * 10 * sizeof(double));
-  c->P.qval = (double*)take(npend * n * 10 * sizeof(double));
-  if ((size_t)(p - (char*)c->state.p) > c->state.n) {
-    // the rounding overflowed the first estimate: reallocate with slack
-    r = dev_alloc(c->state, bytes_state + 16 * 256);
-    if (r) return r;
-    p = (char*)c->state.p;
-    c->P.pos_wl = (double4*)take(n * sizeof(double4));
-    c->P.dir_cos = (double4*)take(n * sizeof(double4));
-    c->P.prev_init = (double4*)take(n * sizeof(double4));
-    c->P.out_in0 = (double4*)take(n * sizeof(double4));
-    c->P.nrm_t = (double4*)take(n * sizeof(double4));
-    c->P.pend = (double4*)take(n * sizeof(double4));
-    c->P.meta = (uint4*)take(n * sizeof(uint4));
-    c->P.cur = (uint4*)take(n * sizeof(uint4));
-    c->P.misc = (uint4*)take(n * sizeof(uint4));
-    c->P.pkey = (uint32_t*)take(npend * n * sizeof(uint32_t));
-    c->P.qkey = (uint32_t*)take(npend * n * sizeof(uint32_t));
-    c->P.pval = (double*)take(npend * n * 10 * sizeof(double));
-    c->P.qval = (double*)take(npend * n * 10 * sizeof(double));
+  {
+    char* p = (char*)c->state.p;
+    void* arr[13];
+    for (int i = 0; i < 13; ++i) { arr[i] = p; p += (sizes[i] + 255) / 256 * 256; }
+    c->P.pos_wl = (double4*)arr[0];
+    c->P.dir_cos = (double4*)arr[1];
+    c->P.prev_init = (double4*)arr[2];
+    c->P.out_in0 = (double4*)arr[3];
+    c->P.nrm_t = (double4*)arr[4];
+    c->P.pend = (double4*)arr[5];
+    c->P.meta = (uint4*)arr[6];
+    c->P.cur = (uint4*)arr[7];
+    c->P.misc = (uint4*)arr[8];
+    c->P.pkey = (uint32_t*)arr[9];
+    c->P.qkey = (uint32_t*)arr[10];
+    c->P.pval = (double*)arr[11];
+    c->P.qval = (double*)arr[12];
   }
-  for (int i = 0; i < 2; ++i) if ((r = dev_alloc(c->rays[i], n * sizeof(RayRec)))) return r;
+  if ((r = dev_alloc(c->rays, n * sizeof(RayRec)))) return r;
   if ((r = dev_alloc(c->hits, n * sizeof(HitRec)))) return r;
+  for (int i = 0; i < 2; ++i) if ((r = dev_alloc(c->lists[i], n * sizeof(uint32_t)))) return r;
+  if ((r = dev_alloc(c->emitted, n))) return r;
+  if ((r = dev_alloc(c->block_emits, max_blocks * sizeof(uint32_t)))) return r;
+  if ((r = dev_alloc(c->block_offsets, max_blocks * sizeof(uint32_t)))) return r;
   if ((r = dev_alloc(c->counters, sizeof(Counters)))) return r;
 
-  // tallies
-  const size_t n_global = 12, n_rcv = (size_t)c->nrcv * 40, n_samp = (size_t)c->nsamp * 4;
+  // tallies (striped where many waves meet, see Tallies)
+  auto pow2_stripes = [](size_t per_stripe_doubles) {
+    size_t k = 256;
+    while (k > 1 && k * per_stripe_doubles > (size_t(1) << 21)) k >>= 1;
+    return (uint32_t)k;
+  };
+  const uint32_t sg = 256, sr = pow2_stripes((size_t)c->nrcv * 40),
+                 ss = pow2_stripes((size_t)c->nsamp * 5);
+  const size_t n_global = (size_t)sg * 16, n_rcv = (size_t)sr * c->nrcv * 40,
+               n_samp = (size_t)ss * c->nsamp * 4;
   const size_t n_sxr = (size_t)c->nsamp * c->nrcv * 40, n_prim = (size_t)c->nprim * 40;
-  const size_t tally_doubles = n_global + n_rcv + n_samp + n_sxr + n_prim;
-  const size_t tally_bytes = tally_doubles * sizeof(double) + (size_t)c->nsamp * 8 + 64;
+  const size_t n_u64 = (size_t)sg * 16 + (size_t)ss * c->nsamp;
+  const size_t tally_bytes = (n_global + n_rcv + n_samp + n_sxr + n_prim + n_u64) * 8 + 64;
   if ((r = dev_alloc(c->tallies, tally_bytes))) return r;
   c->tally_bytes = tally_bytes;
   double* tb = (double*)c->tallies.p;
   c->T.global = tb;
-  c->T.rcv = tb + n_global;
+  c->T.rcv = c->T.global + n_global;
   c->T.samp = c->T.rcv + n_rcv;
   c->T.sxr = c->T.samp + n_samp;
   c->T.prim = c->T.sxr + n_sxr;
-  c->T.samp_count = (unsigned long long*)(c->T.prim + n_prim);
+  c->T.realisations = (unsigned long long*)(c->T.prim + n_prim);
+  c->T.samp_count = c->T.realisations + (size_t)sg * 16;
+  c->T.mask_g = sg - 1;
+  c->T.mask_r = sr - 1;
+  c->T.mask_s = ss - 1;
+  c->T.nrcv = c->nrcv;
+  c->T.nsamp = c->nsamp;
   HIP_TRY(hipMemsetAsync(c->tallies.p, 0, tally_bytes, c->stream));
   HIP_TRY(hipMemsetAsync(c->counters.p, 0, sizeof(Counters), c->stream));
 
@@ -1236,52 +1361,82 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
   prm.npend = npend;
 
   Counters* ctr = (Counters*)c->counters.p;
-  RayRec* rq[2] = {(RayRec*)c->rays[0].p, (RayRec*)c->rays[1].p};
-  HitRec* hq = (HitRec*)c->hits.p;
-  unsigned int* qcount[2] = {&ctr->queue_count[0], &ctr->queue_count[1]};
+  RayRec* rays = (RayRec*)c->rays.p;
+  HitRec* hits = (HitRec*)c->hits.p;
+  uint32_t* lists[2] = {(uint32_t*)c->lists[0].p, (uint32_t*)c->lists[1].p};
+  uint8_t* emitted = (uint8_t*)c->emitted.p;
+  uint32_t* block_emits = (uint32_t*)c->block_emits.p;
+  uint32_t* block_offsets = (uint32_t*)c->block_offsets.p;
 
   float ms_trace = 0, ms_shade = 0;
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  const uint32_t gblocks = slots / BLOCK;
-  k_generate<<<gblocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, rq[0], qcount[0], ctr);
+  // the first realisations fill slots [0, count): the list is the identity
+  uint32_t count = (uint32_t)std::min<uint64_t>(a->realisations, slots);
+  k_generate<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, rays,
+                                                                   count);
   HIP_TRY(hipGetLastError());
-  int cur = 0;
+  const uint32_t* list = nullptr;
+  int next = 0;
   uint32_t iter = 0;
-  unsigned int count = 0;
-  HIP_TRY(hipMemcpyAsync(&count, qcount[0], sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  uint64_t segments = 0;  // one per traced ray
   while (count) {
     if (iter >= 1000000u) {
       set_err("ssolh_solve: the wavefront did not drain after %u iterations", iter);
       return SSOLH_ERR_LIMIT;
     }
-    const int nxt = cur ^ 1;
-    HIP_TRY(hipMemsetAsync(qcount[nxt], 0, sizeof(unsigned int), c->stream));
     const uint32_t blocks = (count + BLOCK - 1) / BLOCK;
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-    k_trace<<<blocks, BLOCK, 0, c->stream>>>(c->scn, rq[cur], hq, qcount[cur], &ctr->segments);
+    k_trace<<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-    k_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, rq[cur], hq, qcount[cur],
-                                             rq[nxt], qcount[nxt], ctr);
+    k_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count, rays, hits,
+                                             emitted, block_emits, ctr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-    HIP_TRY(hipMemcpyAsync(&count, qcount[nxt], sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+    k_scan<<<1, SCAN_THREADS, 0, c->stream>>>(block_emits, blocks, block_offsets, &ctr->total);
+    HIP_TRY(hipGetLastError());
+    k_scatter<<<blocks, BLOCK, 0, c->stream>>>(list, count, emitted, block_offsets, lists[next]);
+    HIP_TRY(hipGetLastError());
+    unsigned int total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, &ctr->total, sizeof(total), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     float t1 = 0, t2 = 0;
     (void)hipEventElapsedTime(&t1, c->ev[1], c->ev[2]);
     (void)hipEventElapsedTime(&t2, c->ev[2], c->ev[3]);
     ms_trace += t1;
     ms_shade += t2;
-    cur = nxt;
+    if (total > count) {
+      set_err("ssolh_solve: work list grew from %u to %u entries", count, total);
+      return SSOLH_ERR_LIMIT;
+    }
+    list = lists[next];
+    next ^= 1;
+    segments += count;
+    count = total;
     ++iter;
   }
-  float total = 0;
+  // fold the stripes
+  auto fold = [&](auto* base, size_t n, uint32_t stripes) -> int {
+    if (stripes > 1 && n) {
+      k_fold<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(base, n, stripes);
+      HIP_TRY(hipGetLastError());
+    }
+    return 0;
+  };
+  if ((r = fold(c->T.global, 16, sg))) return r;
+  if ((r = fold(c->T.realisations, 16, sg))) return r;
+  if ((r = fold(c->T.rcv, (size_t)c->nrcv * 40, sr))) return r;
+  if ((r = fold(c->T.samp, (size_t)c->nsamp * 4, ss))) return r;
+  if ((r = fold(c->T.samp_count, (size_t)c->nsamp, ss))) return r;
+  float total_ms = 0;
   HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  (void)hipEventElapsedTime(&total, c->ev[0], c->ev[3]);
+  (void)hipEventElapsedTime(&total_ms, c->ev[0], c->ev[3]);
   HIP_TRY(hipMemcpy(&c->last, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
-  c->kernel_ms = total;
+  HIP_TRY(hipMemcpy(&c->realisations, c->T.realisations, sizeof(c->realisations),
+                    hipMemcpyDeviceToHost));
+  c->segments = segments;
+  c->kernel_ms = total_ms;
   c->trace_ms = ms_trace;
   c->shade_ms = ms_shade;
   c->iterations = iter;
@@ -1301,9 +1456,9 @@ SSOLH_API int ssolh_tallies_download(ssolh_context* c, struct ssolh_tallies* o) 
   if (o->sampled_x_receivers && n_sxr)
     HIP_TRY(hipMemcpy(o->sampled_x_receivers, c->T.sxr, n_sxr * 8, hipMemcpyDeviceToHost));
   if (o->primitives && n_prim) HIP_TRY(hipMemcpy(o->primitives, c->T.prim, n_prim * 8, hipMemcpyDeviceToHost));
-  o->realisation_count = c->last.realisations;
+  o->realisation_count = c->realisations;
   o->failed_count = c->last.failed;
-  o->segments = c->last.segments;
+  o->segments = c->segments;
   o->pending_spills = c->last.spills;
   o->aborted = c->last.aborted;
   o->iterations = c->iterations;
@@ -1335,17 +1490,47 @@ SSOLH_API int ssolh_test_quadric_hits(ssolh_context* c, int quadric, const doubl
 }
 
 SSOLH_API int ssolh_test_sun_dirs(ssolh_context* c, const struct ssolh_sun* sun, uint32_t n,
-                                  const double* u, double* dir) {
-  if (!c || !sun || !n) return SSOLH_ERR_ARG;
+                                  const double* u, uint32_t nu, double* dir, uint32_t* used) {
+  if (!c || !sun || !n || !nu || !u || !dir || !used) return SSOLH_ERR_ARG;
   HIP_TRY(hipSetDevice(c->device));
   double *d_u, *d_o;
-  HIP_TRY(hipMalloc(&d_u, n * 16)); HIP_TRY(hipMalloc(&d_o, n * 24));
-  HIP_TRY(hipMemcpy(d_u, u, n * 16, hipMemcpyHostToDevice));
-  k_test_sun<<<(n + 127) / 128, 128, 0, c->stream>>>(*sun, n, d_u, d_o);
+  uint32_t* d_used;
+  HIP_TRY(hipMalloc(&d_u, (size_t)n * nu * 8)); HIP_TRY(hipMalloc(&d_o, n * 24));
+  HIP_TRY(hipMalloc(&d_used, n * 4));
+  HIP_TRY(hipMemcpy(d_u, u, (size_t)n * nu * 8, hipMemcpyHostToDevice));
+  k_test_sun<<<(n + 127) / 128, 128, 0, c->stream>>>(*sun, n, d_u, nu, d_o, d_used);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(dir, d_o, n * 24, hipMemcpyDeviceToHost));
-  (void)hipFree(d_u); (void)hipFree(d_o);
+  HIP_TRY(hipMemcpy(used, d_used, n * 4, hipMemcpyDeviceToHost));
+  (void)hipFree(d_u); (void)hipFree(d_o); (void)hipFree(d_used);
+  return 0;
+}
+
+SSOLH_API int ssolh_test_bsdf(ssolh_context* c, const struct ssolh_material* m, double eta_out,
+                              double eta_in, double slab_eta, double slab_absorption, uint32_t n,
+                              const double* wi, const double* normal, const double* u, uint32_t nu,
+                              double* wo, double* weight, int* lobe) {
+  if (!c || !m || !n || !nu || !wi || !normal || !u || !wo || !weight || !lobe) return SSOLH_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->device));
+  double *d_wi, *d_n, *d_u, *d_wo, *d_w;
+  int* d_l;
+  HIP_TRY(hipMalloc(&d_wi, n * 24)); HIP_TRY(hipMalloc(&d_n, n * 24));
+  HIP_TRY(hipMalloc(&d_u, (size_t)n * nu * 8)); HIP_TRY(hipMalloc(&d_wo, n * 24));
+  HIP_TRY(hipMalloc(&d_w, n * 8)); HIP_TRY(hipMalloc(&d_l, n * 4));
+  HIP_TRY(hipMemcpy(d_wi, wi, n * 24, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_n, normal, n * 24, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_u, u, (size_t)n * nu * 8, hipMemcpyHostToDevice));
+  k_test_bsdf<<<(n + 127) / 128, 128, 0, c->stream>>>(
+      *m, make_double4(eta_out, eta_in, slab_eta, slab_absorption), n, d_wi, d_n, d_u, nu, d_wo,
+      d_w, d_l);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(wo, d_wo, n * 24, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(weight, d_w, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(lobe, d_l, n * 4, hipMemcpyDeviceToHost));
+  (void)hipFree(d_wi); (void)hipFree(d_n); (void)hipFree(d_u); (void)hipFree(d_wo);
+  (void)hipFree(d_w); (void)hipFree(d_l);
   return 0;
 }
 

@@ -80,22 +80,27 @@ SD void threefry4x64_20(const uint64_t c[4], const uint64_t k[4], uint64_t o[4])
 
 // Sequential draws of one realisation: block b of path p is Threefry(key,
 // (p, b, 0, 0)); each block gives four 53-bit uniforms in [0, 1).
+// The four words of a block are consumed in order; they are kept in named
+// registers (a dynamically indexed array would live in scratch or LDS).
 struct PathRng {
   uint64_t key, path;
   uint32_t block;
   uint32_t avail;
-  uint64_t buf[4];
+  uint64_t b0, b1, b2, b3;
 
   SD void init(uint64_t k, uint64_t p, uint32_t b) { key = k; path = p; block = b; avail = 0; }
   SD double next() {
     if (!avail) {
       const uint64_t c[4] = {path, (uint64_t)block, 0, 0};
       const uint64_t kk[4] = {key, TAG_PATH, 0, 0};
-      threefry4x64_20(c, kk, buf);
+      uint64_t o[4];
+      threefry4x64_20(c, kk, o);
+      b0 = o[0]; b1 = o[1]; b2 = o[2]; b3 = o[3];
       ++block;
       avail = 4;
     }
-    const uint64_t x = buf[4 - avail--];
+    const uint64_t x = avail == 4 ? b0 : (avail == 3 ? b1 : (avail == 2 ? b2 : b3));
+    --avail;
     return (double)(x >> 11) * (1.0 / 9007199254740992.0);
   }
 };
@@ -165,7 +170,8 @@ SD double buie_pdf(const double* p, double theta) {
   return p[4] * phi * sin(theta);
 }
 
-SD d3 sample_sun(const ssolh_sun& sun, PathRng& rng) {
+template <class Rng>
+SD d3 sample_sun(const ssolh_sun& sun, Rng& rng) {
   double st, ct, phi;
   d3 loc;
   switch (sun.type) {
@@ -339,9 +345,10 @@ SD d3 sample_microfacet(int distrib, double m, double u1, double u2, d3 n) {
 SD d3 reflect(d3 wi, d3 n, double cos_i) { return n * (2 * cos_i) - wi; }
 
 // wi points away from the surface on n's side. Returns the MC weight.
+template <class Rng>
 SD double bsdf_sample(const ssolh_material& m, double eta_out, double eta_in,
                       double slab_eta, double slab_abs, d3 wi, d3 n,
-                      PathRng& rng, d3& wo, int& lobe) {
+                      Rng& rng, d3& wo, int& lobe) {
   const double cos_i = dot(wi, n);
   lobe = LOBE_REFLECT;
   switch (m.type) {

@@ -110,29 +110,107 @@ def test_quadric_hits_match_oracle(gpu, ctx, q, data):
     assert checked > n // 4
 
 
-@pytest.mark.parametrize("kind,param", [(1, 4.65e-3), (1, 1.0), (2, 2.51e-3), (2, 0.01)])
-def test_sun_dirs_match_oracle(gpu, ctx, kind, param):
+class Sun(C.Structure):
+    _fields_ = [("type", C.c_int32), ("wl_spectrum", C.c_int32), ("dir", C.c_double * 3),
+                ("basis", C.c_double * 9), ("dni", C.c_double), ("param", C.c_double * 8),
+                ("wl_cdf_total", C.c_double)]
+
+
+def _gpu_sun(gpu, ctx, kind, param, sun_dir, u):
+    basis = O.basis(sun_dir)
+    p = np.zeros(8)
+    if kind == 1:
+        p[0] = np.sin(param) ** 2
+    elif kind == 2:
+        p[0] = param
+    elif kind == 3:
+        p[:] = O.buie_params(param)
+    s = Sun(kind, 0, (C.c_double * 3)(*sun_dir), (C.c_double * 9)(*basis.T.reshape(-1)), 1000.0,
+            (C.c_double * 8)(*p), 0.0)
+    n, nu = u.shape
+    out = np.zeros((n, 3))
+    used = np.zeros(n, dtype=np.uint32)
+    gpu.ssolh_test_sun_dirs.argtypes = [C.c_void_p, C.POINTER(Sun), C.c_uint32,
+                                        C.POINTER(C.c_double), C.c_uint32,
+                                        C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
+    assert gpu.ssolh_test_sun_dirs(ctx, C.byref(s), n, _p(u), nu, _p(out),
+                                   _p(used, C.c_uint32)) == 0
+    return out, used
+
+
+@pytest.mark.parametrize("kind,param,nu", [(1, 4.65e-3, 2), (1, 1.0, 2), (2, 2.51e-3, 2),
+                                           (2, 0.01, 2), (3, 0.05, 64), (3, 0.2, 64)])
+def test_sun_dirs_match_oracle(gpu, ctx, kind, param, nu):
+    """Pillbox, Gaussian and Buie samplers on the same uniforms: tolerance
+    1e-12 on each component and the same number of uniforms consumed."""
     import ssol_ctypes  # noqa: F401  (loads the product library first)
     rng = np.random.default_rng(3)
     n = 256
-    u = np.ascontiguousarray(rng.uniform(1e-12, 1, (n, 2)))
+    u = np.ascontiguousarray(rng.uniform(1e-12, 1, (n, nu)))
     sun_dir = np.array([0.3, -0.2, -0.93])
     sun_dir /= np.linalg.norm(sun_dir)
-    basis = O.basis(sun_dir)
-    p = np.zeros(8)
-    p[0] = np.sin(param) ** 2 if kind == 1 else param
-
-    class Sun(C.Structure):
-        _fields_ = [("type", C.c_int32), ("wl_spectrum", C.c_int32), ("dir", C.c_double * 3),
-                    ("basis", C.c_double * 9), ("dni", C.c_double), ("param", C.c_double * 8),
-                    ("wl_cdf_total", C.c_double)]
-    s = Sun(kind, 0, (C.c_double * 3)(*sun_dir), (C.c_double * 9)(*basis.T.reshape(-1)), 1000.0,
-            (C.c_double * 8)(*p), 0.0)
-    out = np.zeros((n, 3))
-    gpu.ssolh_test_sun_dirs.argtypes = [C.c_void_p, C.POINTER(Sun), C.c_uint32,
-                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
-    assert gpu.ssolh_test_sun_dirs(ctx, C.byref(s), n, _p(u), _p(out)) == 0
+    out, used = _gpu_sun(gpu, ctx, kind, param, sun_dir, u)
     for i in range(n):
-        want, used = O.sun_sample(kind, param, sun_dir, u[i])
-        assert used == 2
+        want, nused = O.sun_sample(kind, param, sun_dir, u[i])
+        assert used[i] == nused
         assert np.allclose(out[i], want, rtol=0, atol=1e-12)
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("distribution", C.c_int32), ("medium_in", C.c_int32),
+                ("medium_out", C.c_int32), ("normal_from_ng", C.c_int32), ("pad", C.c_int32),
+                ("reflectivity", C.c_double), ("roughness", C.c_double),
+                ("thickness", C.c_double), ("slab_eta", C.c_double * 2),
+                ("slab_absorption", C.c_double * 2)]
+
+
+BSDF_CASES = [
+    ("mirror", dict(reflectivity=0.9), {}),
+    ("beckmann", dict(reflectivity=0.92, roughness=0.05, distribution=0), {}),
+    ("pillbox", dict(reflectivity=0.8, roughness=0.2, distribution=1), {}),
+    ("matte", dict(reflectivity=0.3), {}),
+    ("dielectric", {}, dict(eta_out=1.0, eta_in=1.5)),
+    ("dielectric_tir", {}, dict(eta_out=1.5, eta_in=1.0)),
+    ("thin", dict(thickness=3e-3), dict(slab_eta=1.5, slab_abs=50.0)),
+]
+
+
+@pytest.mark.parametrize("name,mat,eta", BSDF_CASES, ids=[c[0] for c in BSDF_CASES])
+def test_bsdf_samples_match_oracle(gpu, ctx, name, mat, eta):
+    """BSDF samples from fixed uniforms: outgoing direction within 1e-12,
+    weight within 1e-12, same lobe (the glossy models, the dielectric and
+    the thin slab are the build's own physics; their per-sample parity with
+    the reference is unpinned, the oracle and the GPU share the model)."""
+    import ssol_scene as S
+    kind = {"mirror": "mirror", "beckmann": "mirror", "pillbox": "mirror", "matte": "matte",
+            "dielectric": "dielectric", "dielectric_tir": "dielectric", "thin": "thin"}[name]
+    m = S.Material(kind, **mat)
+    rng = np.random.default_rng(11)
+    n, nu = 512, 64
+    N = rng.normal(size=(n, 3))
+    N /= np.linalg.norm(N, axis=1)[:, None]
+    wi = rng.normal(size=(n, 3))
+    wi /= np.linalg.norm(wi, axis=1)[:, None]
+    flip = np.sum(wi * N, axis=1) < 0.02
+    wi[flip] = -wi[flip] + 0.1 * N[flip]
+    wi /= np.linalg.norm(wi, axis=1)[:, None]
+    u = rng.uniform(0, 1, (n, nu))
+    e = dict(eta_out=1.0, eta_in=1.0, slab_eta=1.0, slab_abs=0.0)
+    e.update(eta)
+    cm = Material(m.type_id, m.distribution, -1, -1, 0, 0, m.reflectivity, m.roughness,
+                  m.thickness, (C.c_double * 2)(0, 0), (C.c_double * 2)(0, 0))
+    wo = np.zeros((n, 3))
+    w = np.zeros(n)
+    lobe = np.zeros(n, dtype=np.int32)
+    gpu.ssolh_test_bsdf.argtypes = [C.c_void_p, C.POINTER(Material)] + [C.c_double] * 4 + [
+        C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+        C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    wi_c, N_c, u_c = (np.ascontiguousarray(x) for x in (wi, N, u))
+    assert gpu.ssolh_test_bsdf(ctx, C.byref(cm), e["eta_out"], e["eta_in"], e["slab_eta"],
+                               e["slab_abs"], n, _p(wi_c), _p(N_c), _p(u_c), nu, _p(wo), _p(w),
+                               lobe.ctypes.data_as(C.POINTER(C.c_int))) == 0
+    for i in range(n):
+        ww, wwo, t = O.bsdf_sample(m, wi[i], N[i], u[i], **e)
+        assert abs(w[i] - ww) <= 1e-12, (i, w[i], ww)
+        assert lobe[i] == t, (i, lobe[i], t)
+        assert np.allclose(wo[i], wwo, rtol=0, atol=1e-12), (i, wo[i], wwo)
