@@ -226,6 +226,8 @@ struct ssolh_scene {
   uint32_t nnodes;
   const struct ssolh_bvh_node* nodes;   /* BLASes then TLAS */
   uint32_t tlas_root;
+  uint32_t stack_depth;       /* traversal stack entries a ray can need:
+                                 TLAS depth + deepest BLAS + 2 (<= 48) */
 
   uint32_t ntriangles;
   const struct ssolh_triangle* triangles;

@@ -45,7 +45,7 @@ namespace {
 
 constexpr int BLOCK = 128;           // threads per block (2 waves)
 constexpr int WAVES = BLOCK / 64;
-constexpr int STACK = 48;            // traversal stack entries per thread (LDS)
+constexpr int MAX_STACK = 48;        // traversal stack entries per thread (LDS), largest kernel
 constexpr int SCAN_THREADS = 1024;   // the one block of k_scan
 constexpr uint32_t SENTINEL = 0xFFFFFFFFu;
 constexpr uint32_t LEAF = 0x80000000u;
@@ -383,7 +383,11 @@ __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t is
   return true;
 }
 
-__global__ void __launch_bounds__(BLOCK)
+// STACK: LDS stack entries per lane, at least the scene's stack_depth (the
+// smaller the stack, the more blocks share a CU). Four waves per SIMD caps
+// the kernel at 128 VGPRs; the few values that spill are cold.
+template <int STACK>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_trace(DScene S, const uint32_t* __restrict__ list, uint32_t count,
         const RayRec* __restrict__ rays, HitRec* __restrict__ hits) {
   __shared__ uint32_t lds_stack[STACK * BLOCK];
@@ -411,50 +415,21 @@ k_trace(DScene S, const uint32_t* __restrict__ list, uint32_t count,
   bool in_blas = false;
   uint32_t iss_first = 0;   // (instance, sshape) base of the BLAS being traversed
   const float ftmin = r.tmin;
-  bool done = false;
 
-  uint32_t ref = S.tlas_root;  // inner node reference
-  for (;;) {
-    if (ref == SENTINEL) {
-      // back to the TLAS
-      in_blas = false;
-      fr = make_fray(ow, dw);
-      o = ow;
-      d = dw;
-    } else if (ref & LEAF) {
-      const uint32_t first = ref & 0x0FFFFFFFu;
-      const uint32_t cnt = ((ref >> 28) & 7u) + 1u;
-      if (!in_blas) {
-        // TLAS leaf: one instance, enter its BLAS
-        const ssolh_instance& inst = S.insts[S.tlas_inst[first]];
-        const d3 T = mk(inst.transform[9], inst.transform[10], inst.transform[11]);
-        o = mul(inst.inv_linear, ow - T);
-        d = mul(inst.inv_linear, dw);
-        fr = make_fray(o, d);
-        sh = make_shear(d);
-        iss_first = inst.isshape_first;
-        in_blas = true;
-        if (sp < STACK) { stack[sp * BLOCK] = SENTINEL; ++sp; }
-        ref = S.objects[inst.object].blas_root;
-        continue;
-      }
-      for (uint32_t k = 0; k < cnt; ++k) {
-        const ssolh_triangle tri = S.tris[first + k];
-        const double t = ray_triangle(o, sh, ld3(tri.v0), ld3(tri.v1), ld3(tri.v2));
-        if (!(t >= r.tmin && t < ts.tbest)) continue;
-        d3 ng;
-        bool punched;
-        const uint32_t iss = iss_first + tri.sshape;
-        if (!filter(S, r, ow, dw, iss, tri, t, ts, ng, punched)) continue;
-        ts.tbest = t;
-        ts.iss = iss;
-        ts.prim = tri.prim;
-        ts.ng = ng;
-        ts.punched = punched;
-        if (any_hit) { done = true; break; }
-      }
-      if (done) break;
-    } else {
+  // "while-while" traversal: the lanes of a wave first descend through inner
+  // nodes together, then process their leaves together, which keeps the
+  // node, instance and triangle code paths from interleaving per lane.
+  // SENTINEL (all bits set) reads as a leaf reference and ends the descent.
+  uint32_t ref = S.tlas_root;
+  bool alive = true;
+  auto pop = [&]() -> bool {
+    if (sp == 0) return false;
+    --sp;
+    ref = stack[sp * BLOCK];
+    return true;
+  };
+  while (alive) {
+    while (!(ref & LEAF)) {
       const ssolh_bvh_node n = S.nodes[ref];
       const float tmax = ts.tbest < (double)FLT_MAX ? (float)ts.tbest * 1.0000002f + 1e-30f : FLT_MAX;
       float tn0, tn1;
@@ -468,18 +443,59 @@ k_trace(DScene S, const uint32_t* __restrict__ list, uint32_t count,
         const uint32_t rn = enc(near0 ? 0 : 1), rf = enc(near0 ? 1 : 0);
         if (sp < STACK) { stack[sp * BLOCK] = rf; ++sp; }
         ref = rn;
-        continue;
       } else if (h0) {
         ref = enc(0);
-        continue;
       } else if (h1) {
         ref = enc(1);
-        continue;
+      } else if (!pop()) {
+        alive = false;
+        break;
       }
     }
-    if (sp == 0) break;
-    --sp;
-    ref = stack[sp * BLOCK];
+    if (!alive) break;
+    if (ref == SENTINEL) {
+      // back to the TLAS
+      in_blas = false;
+      fr = make_fray(ow, dw);
+      o = ow;
+      d = dw;
+      alive = pop();
+      continue;
+    }
+    const uint32_t first = ref & 0x0FFFFFFFu;
+    if (!in_blas) {
+      // TLAS leaf: one instance, enter its BLAS
+      const ssolh_instance& inst = S.insts[S.tlas_inst[first]];
+      const d3 T = mk(inst.transform[9], inst.transform[10], inst.transform[11]);
+      o = mul(inst.inv_linear, ow - T);
+      d = mul(inst.inv_linear, dw);
+      fr = make_fray(o, d);
+      sh = make_shear(d);
+      iss_first = inst.isshape_first;
+      in_blas = true;
+      if (sp < STACK) { stack[sp * BLOCK] = SENTINEL; ++sp; }
+      ref = S.objects[inst.object].blas_root;
+      continue;
+    }
+    const uint32_t cnt = ((ref >> 28) & 7u) + 1u;
+    bool done = false;
+    for (uint32_t k = 0; k < cnt; ++k) {
+      const ssolh_triangle tri = S.tris[first + k];
+      const double t = ray_triangle(o, sh, ld3(tri.v0), ld3(tri.v1), ld3(tri.v2));
+      if (!(t >= r.tmin && t < ts.tbest)) continue;
+      d3 ng;
+      bool punched;
+      const uint32_t iss = iss_first + tri.sshape;
+      if (!filter(S, r, ow, dw, iss, tri, t, ts, ng, punched)) continue;
+      ts.tbest = t;
+      ts.iss = iss;
+      ts.prim = tri.prim;
+      ts.ng = ng;
+      ts.punched = punched;
+      if (any_hit) { done = true; break; }
+    }
+    if (done) break;
+    alive = pop();
   }
 
   HitRec h;
@@ -1094,6 +1110,7 @@ struct ssolh_context {
   bool has_scene = false;
   uint32_t nrcv = 0, nsamp = 0;
   uint64_t nprim = 0;
+  uint32_t stack_depth = MAX_STACK;
   // solve buffers
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, counters, tallies;
   PathState P{};
@@ -1221,6 +1238,11 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
             s->nsample_tris, s->ninstances);
     return SSOLH_ERR_ARG;
   }
+  if (s->stack_depth < 2 || s->stack_depth > (uint32_t)MAX_STACK) {
+    set_err("ssolh_scene_upload: traversal stack depth %u outside [2, %d]", s->stack_depth,
+            MAX_STACK);
+    return SSOLH_ERR_LIMIT;
+  }
   std::vector<IssInfo> iss;
   if (build_iss(s, iss)) {
     set_err("ssolh_scene_upload: inconsistent instance / shaded-shape indices");
@@ -1261,6 +1283,7 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   c->nrcv = s->nreceivers;
   c->nsamp = s->nsampled;
   c->nprim = s->nprim_tallies;
+  c->stack_depth = s->stack_depth;
   HIP_TRY(hipStreamSynchronize(c->stream));  // the host arrays may go after return
   c->has_scene = true;
   return 0;
@@ -1386,7 +1409,14 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
     }
     const uint32_t blocks = (count + BLOCK - 1) / BLOCK;
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-    k_trace<<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+    if (c->stack_depth <= 16)
+      k_trace<16><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+    else if (c->stack_depth <= 24)
+      k_trace<24><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+    else if (c->stack_depth <= 32)
+      k_trace<32><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+    else
+      k_trace<MAX_STACK><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     k_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count, rays, hits,

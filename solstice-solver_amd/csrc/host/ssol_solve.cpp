@@ -558,6 +558,9 @@ res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
               tdepth, max_blas);
     return RES_MEM_ERR;
   }
+  f.scn.stack_depth = tdepth + max_blas + 2;
+  log_warning(dev, "ssol_solve: %zu instances, %zu triangles, TLAS depth %u, BLAS depth %u.\n",
+              f.instances.size(), f.tris.size(), tdepth, max_blas);
 
   // sun (src/ssol_sun.c:214-279)
   ssolh_sun& su = f.scn.sun;
