@@ -38,6 +38,7 @@
 
 #include "ssolh.h"
 #include "ssolh_device.hpp"
+#include "ssolh_device32.hpp"
 
 using namespace ssolh;
 
@@ -1015,6 +1016,8 @@ k_scatter(const uint32_t* __restrict__ list, uint32_t count, const uint8_t* __re
   if (pred) next[base + (uint32_t)__popcll(below)] = slot;
 }
 
+#include "ssolh_fp32.hpp"
+
 // Sum of the stripes of a striped tally into stripe 0
 template <typename T>
 __global__ void k_fold(T* base, size_t n, uint32_t stripes) {
@@ -1114,6 +1117,8 @@ struct ssolh_context {
   // solve buffers
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, counters, tallies;
   PathState P{};
+  PathState32 P32{};
+  const ShapeQ* shapeq = nullptr;  // per shape, in scene_bufs
   Tallies T{};
   size_t tally_bytes = 0;
   // last solve
@@ -1151,6 +1156,23 @@ void free_scene(ssolh_context* c) {
   for (auto& b : c->scene_bufs) if (b.p) (void)hipFree(b.p);
   c->scene_bufs.clear();
   c->has_scene = false;
+}
+
+// Inverse of a column-major 3x3 matrix; false when singular
+bool inv3(const double* m, double* o) {
+  const double a = m[0], b = m[3], cc = m[6], d = m[1], e = m[4], f = m[7], g = m[2], h = m[5],
+               i = m[8];
+  const double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+  const double det = a * A + b * B + cc * C;
+  if (det == 0 || !std::isfinite(det)) return false;
+  const double r = 1.0 / det;
+  // rows of the inverse (row-major r0..r2) stored column-major
+  const double inv[3][3] = {{A * r, -(b * i - cc * h) * r, (b * f - cc * e) * r},
+                            {B * r, (a * i - cc * g) * r, -(a * f - cc * d) * r},
+                            {C * r, -(a * h - b * g) * r, (a * e - b * d) * r}};
+  for (int row = 0; row < 3; ++row)
+    for (int col = 0; col < 3; ++col) o[col * 3 + row] = inv[row][col];
+  return true;
 }
 
 // Per (instance, shaded shape) records of the hit path, from the flat scene
@@ -1243,6 +1265,16 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
             MAX_STACK);
     return SSOLH_ERR_LIMIT;
   }
+  std::vector<ShapeQ> shapeq(s->nshapes);
+  for (uint32_t i = 0; i < s->nshapes; ++i) {
+    const double* m = s->shapes[i].qtransform;
+    ShapeQ& q = shapeq[i];
+    if (!inv3(m, q.Rinv)) {
+      set_err("ssolh_scene_upload: singular quadric transform of shape %u", i);
+      return SSOLH_ERR_ARG;
+    }
+    for (int k = 0; k < 3; ++k) q.t[k] = m[9 + k];
+  }
   std::vector<IssInfo> iss;
   if (build_iss(s, iss)) {
     set_err("ssolh_scene_upload: inconsistent instance / shaded-shape indices");
@@ -1269,6 +1301,7 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   r |= upload(c, s->tlas_instances, s->ninstances, &d.tlas_inst);
   r |= upload(c, s->inst_sshapes, s->ninst_sshapes, &d.isshapes);
   r |= upload(c, iss.data(), iss.size(), &d.iss);
+  r |= upload(c, shapeq.data(), shapeq.size(), &c->shapeq);
   r |= upload(c, s->sample_tris, s->nsample_tris, &d.stris);
   r |= upload(c, s->sample_alias, s->nsample_tris, &d.alias);
   if (r) return r;
@@ -1292,10 +1325,11 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
 SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
   if (!c || !a || !a->realisations) { set_err("ssolh_solve: invalid arguments"); return SSOLH_ERR_ARG; }
   if (!c->has_scene) { set_err("ssolh_solve: no scene uploaded"); return SSOLH_ERR_ARG; }
-  if (a->precision != SSOLH_FP64) {
-    set_err("ssolh_solve: precision %u not available", a->precision);
+  if (a->precision != SSOLH_FP64 && a->precision != SSOLH_FP32) {
+    set_err("ssolh_solve: unknown precision %u", a->precision);
     return SSOLH_ERR_ARG;
   }
+  const bool fp32 = a->precision == SSOLH_FP32;
   HIP_TRY(hipSetDevice(c->device));
   const uint32_t npend = a->pending_entries ? a->pending_entries : 4;
   if (npend > MAX_PEND) { set_err("ssolh_solve: at most %d pending entries", MAX_PEND); return SSOLH_ERR_LIMIT; }
@@ -1306,14 +1340,15 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
   const uint32_t slots = (uint32_t)slots64;
   const uint32_t max_blocks = slots / BLOCK;
 
-  // path state: 9 arrays of 16/32-byte records and the pending entries,
+  // path state: 9 arrays of 16- or 32-byte records and the pending entries,
   // each array starting on a 256-byte boundary
   const size_t n = slots;
-  const size_t sizes[13] = {n * sizeof(double4), n * sizeof(double4), n * sizeof(double4),
-                            n * sizeof(double4), n * sizeof(double4), n * sizeof(double4),
+  const size_t real = fp32 ? sizeof(float) : sizeof(double);
+  const size_t sizes[13] = {n * 4 * real, n * 4 * real, n * 4 * real, n * 4 * real,
+                            n * 4 * real, n * 4 * real,
                             n * sizeof(uint4), n * sizeof(uint4), n * sizeof(uint4),
                             npend * n * sizeof(uint32_t), npend * n * sizeof(uint32_t),
-                            npend * n * 10 * sizeof(double), npend * n * 10 * sizeof(double)};
+                            npend * n * 10 * real, npend * n * 10 * real};
   size_t bytes_state = 0;
   for (size_t sz : sizes) bytes_state += (sz + 255) / 256 * 256;
   int r = dev_alloc(c->state, bytes_state);
@@ -1335,9 +1370,22 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
     c->P.qkey = (uint32_t*)arr[10];
     c->P.pval = (double*)arr[11];
     c->P.qval = (double*)arr[12];
+    c->P32.pos = (float4*)arr[0];
+    c->P32.dir_wl = (float4*)arr[1];
+    c->P32.prev_init = (float4*)arr[2];
+    c->P32.out_in0 = (float4*)arr[3];
+    c->P32.nrm_t = (float4*)arr[4];
+    c->P32.pend_cos = (float4*)arr[5];
+    c->P32.meta = (uint4*)arr[6];
+    c->P32.cur = (uint4*)arr[7];
+    c->P32.misc = (uint4*)arr[8];
+    c->P32.pkey = (uint32_t*)arr[9];
+    c->P32.qkey = (uint32_t*)arr[10];
+    c->P32.pval = (float*)arr[11];
+    c->P32.qval = (float*)arr[12];
   }
-  if ((r = dev_alloc(c->rays, n * sizeof(RayRec)))) return r;
-  if ((r = dev_alloc(c->hits, n * sizeof(HitRec)))) return r;
+  if ((r = dev_alloc(c->rays, n * (fp32 ? sizeof(RayRec32) : sizeof(RayRec))))) return r;
+  if ((r = dev_alloc(c->hits, n * (fp32 ? sizeof(HitRec32) : sizeof(HitRec))))) return r;
   for (int i = 0; i < 2; ++i) if ((r = dev_alloc(c->lists[i], n * sizeof(uint32_t)))) return r;
   if ((r = dev_alloc(c->emitted, n))) return r;
   if ((r = dev_alloc(c->block_emits, max_blocks * sizeof(uint32_t)))) return r;
@@ -1395,8 +1443,12 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
   // the first realisations fill slots [0, count): the list is the identity
   uint32_t count = (uint32_t)std::min<uint64_t>(a->realisations, slots);
-  k_generate<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, rays,
-                                                                   count);
+  if (fp32)
+    k_generate32<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
+        c->scn, c->shapeq, c->P32, prm, c->T, (RayRec32*)c->rays.p, count);
+  else
+    k_generate<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, rays,
+                                                                     count);
   HIP_TRY(hipGetLastError());
   const uint32_t* list = nullptr;
   int next = 0;
@@ -1409,18 +1461,36 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
     }
     const uint32_t blocks = (count + BLOCK - 1) / BLOCK;
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-    if (c->stack_depth <= 16)
-      k_trace<16><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
-    else if (c->stack_depth <= 24)
-      k_trace<24><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
-    else if (c->stack_depth <= 32)
-      k_trace<32><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
-    else
-      k_trace<MAX_STACK><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+    if (fp32) {
+      RayRec32* r32 = (RayRec32*)c->rays.p;
+      HitRec32* h32 = (HitRec32*)c->hits.p;
+      if (c->stack_depth <= 16)
+        k_trace32<16><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list, count, r32, h32);
+      else if (c->stack_depth <= 24)
+        k_trace32<24><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list, count, r32, h32);
+      else if (c->stack_depth <= 32)
+        k_trace32<32><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list, count, r32, h32);
+      else
+        k_trace32<MAX_STACK><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list, count, r32, h32);
+    } else {
+      if (c->stack_depth <= 16)
+        k_trace<16><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+      else if (c->stack_depth <= 24)
+        k_trace<24><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+      else if (c->stack_depth <= 32)
+        k_trace<32><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+      else
+        k_trace<MAX_STACK><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-    k_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count, rays, hits,
-                                             emitted, block_emits, ctr);
+    if (fp32)
+      k_shade32<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, c->P32, prm, c->T, list, count,
+                                                 (RayRec32*)c->rays.p, (const HitRec32*)c->hits.p,
+                                                 emitted, block_emits, ctr);
+    else
+      k_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count, rays, hits,
+                                               emitted, block_emits, ctr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[3], c->stream));
     k_scan<<<1, SCAN_THREADS, 0, c->stream>>>(block_emits, blocks, block_offsets, &ctr->total);

@@ -264,7 +264,7 @@ class Run:
             self.e = self.o
         else:
             import ssol_ctypes as L
-            self.lib = L.Scene(scene)
+            self.lib = L.Scene(scene, precision="fp32" if backend == "gpu32" else "fp64")
             self.e = self.lib.solve(n, seed=seed, counter=counter, max_failed=max_failed)
             self.res = self.e.res
 
