@@ -126,6 +126,8 @@ def main(argv=None):
     ap.add_argument("--config", choices=("c1", "c2", "c3"), default="c3")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--path-slots", type=int, default=0)
+    ap.add_argument("--graph", type=int, default=0,
+                    help="wavefront iterations per HIP graph launch (0: host-driven)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth yardstick")
@@ -146,7 +148,8 @@ def main(argv=None):
                 "c3": "config3_10k_heliostat_beam_down"}[args.config]
     print(f"[bench] building {spec.name} ({len(spec.instances)} instances)", file=sys.stderr,
           flush=True)
-    lib = L.Scene(spec, precision=args.precision, path_slots=args.path_slots)
+    lib = L.Scene(spec, precision=args.precision, path_slots=args.path_slots,
+                  graph_iterations=args.graph)
     B = args.realisations
 
     for w in range(args.warmup):
@@ -172,9 +175,14 @@ def main(argv=None):
     value = args.steps * B / total / 1e6
     g = est.global_()
 
-    # roofline of the dominant kernel, from in-stream HIP events
+    # roofline of the dominant kernel, from in-stream HIP events (the
+    # host-driven loop records them; a graph-captured run has none)
     fp32 = args.precision == "fp32"
-    if trace_ms >= shade_ms:
+    if trace_ms <= 0 and shade_ms <= 0:
+        print("bench.py: no per-kernel times (graph mode); run with --graph 0 for the roofline",
+              file=sys.stderr)
+        trace_ms = shade_ms = float("nan")
+    if not (shade_ms > trace_ms):
         kernel, ms, per_seg = "k_trace", trace_ms, (BYTES_TRACE_FP32 if fp32 else BYTES_TRACE_FP64)
     else:
         kernel, ms, per_seg = "k_shade", shade_ms, (BYTES_SHADE_FP32 if fp32 else BYTES_SHADE_FP64)

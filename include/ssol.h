@@ -640,6 +640,36 @@ struct ssol_solve_stats {
 SSOL_API res_T ssol_estimator_get_solve_stats
   (const struct ssol_estimator* estimator, struct ssol_solve_stats* stats);
 
+/* Wavefront iterations captured per HIP graph launch (0, the default: one
+ * host round trip per iteration). */
+SSOL_API res_T ssol_device_set_graph_iterations
+  (struct ssol_device* dev, const unsigned iterations);
+
+/* Several sun positions over one scene in one solve. Sun j runs
+ * realisations[j] realisations; its global estimates (same definitions as
+ * ssol_estimator_get_mc_global) land in globals[j] and its realisation
+ * count in counts[j] (counts may be NULL). Every sun must use the spectrum
+ * object of the sun attached to the scene. The RNG streams continue across
+ * the suns from `rng`, so the batch equals consecutive single-sun solves
+ * with the counter advanced by each count. */
+SSOL_API res_T ssol_solve_sun_batch
+  (struct ssol_scene* scn, const struct ssp_rng* rng, const size_t nsuns,
+   struct ssol_sun* const* suns, const size_t* realisations,
+   struct ssol_mc_global* globals, size_t* counts);
+
+/* Autotuner: times probe solves of `probe` realisations over candidate
+ * wavefront widths (path slots) and graph lengths, keeps the fastest as the
+ * device's setting and reports it. */
+struct ssol_autotune_result {
+  unsigned path_slots;
+  unsigned graph_iterations;
+  double paths_per_second;   /* of the kept candidate */
+  unsigned candidates;       /* number of candidates timed */
+};
+SSOL_API res_T ssol_device_autotune
+  (struct ssol_device* dev, struct ssol_scene* scn, const size_t probe,
+   struct ssol_autotune_result* out);
+
 #ifdef __cplusplus
 }
 #endif

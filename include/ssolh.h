@@ -263,7 +263,32 @@ struct ssolh_solve_args {
   uint32_t precision;         /* enum ssolh_precision */
   uint32_t path_slots;        /* 0 <=> automatic */
   uint32_t pending_entries;   /* per-path pending tally entries, 0 => 4 */
+  uint32_t graph_iterations;  /* 0: iterations driven from the host; n > 0:
+                                 n (rounded up to even) iterations captured
+                                 in a HIP graph and replayed until done */
+};
+
+/* Several suns over the uploaded scene in one solve: sun j runs
+ * realisations[j] realisations (rng counters continue across suns). The
+ * scene's receivers, sampled and per-primitive tallies accumulate over all
+ * suns; the global fields and the realisation count are kept per sun. */
+struct ssolh_batch_args {
+  uint32_t nsuns;
   uint32_t pad__;
+  const struct ssolh_sun* suns;     /* [nsuns], same spectrum pool as the scene */
+  const uint64_t* realisations;     /* [nsuns] */
+  uint64_t max_failed;
+  uint64_t rng_key;
+  uint64_t rng_counter;
+  uint32_t precision;
+  uint32_t path_slots;
+  uint32_t pending_entries;
+  uint32_t graph_iterations;
+};
+
+struct ssolh_batch_result {
+  double global[6][2];        /* SSOLH_GLOBAL_FIELDS order, {sum w, sum w2} */
+  uint64_t realisation_count;
 };
 
 /* Number of doubles of each tally array (pairs {sum w, sum w2}) */
@@ -300,6 +325,11 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* ctx, const struct ssolh_scene* s
 
 /* Run the realisations of one solve over the uploaded scene */
 SSOLH_API int ssolh_solve(ssolh_context* ctx, const struct ssolh_solve_args* args);
+
+/* Batched solve (see ssolh_batch_args); out: [nsuns] */
+SSOLH_API int ssolh_solve_batch
+  (ssolh_context* ctx, const struct ssolh_batch_args* args,
+   struct ssolh_batch_result* out);
 
 /* Copy the tallies of the last solve into host arrays the caller sized
  * from the uploaded scene's tally layout */

@@ -164,8 +164,9 @@ struct PathState {
 struct Counters {
   unsigned long long failed;
   unsigned long long spills;
+  unsigned long long segments;      // traced rays (added by k_scan, one thread)
   unsigned int aborted;
-  unsigned int total;               // length of the next work list
+  unsigned int pad;
 };
 
 // Tallies that every wave updates (global fields, counts) or that few keys
@@ -174,15 +175,15 @@ struct Counters {
 // stripes are folded into stripe 0 after the solve. Keys that are many and
 // scattered (sampled x receiver, per primitive) have one copy.
 struct Tallies {
-  double* global;   // [mask_g+1][16]: 6 fields x {w, w2}, one 128-byte line per stripe
-  unsigned long long* realisations;  // [mask_g+1][16]
+  double* global;   // [mask_g+1][njobs][16]: 6 fields x {w, w2}, then the
+                    // realisation count (exact below 2^53); one 128-byte line
   double* rcv;      // [mask_r+1][nrcv][2][10][2]
   double* samp;     // [mask_s+1][nsamp][2][2]
   unsigned long long* samp_count;    // [mask_s+1][nsamp]
   double* sxr;      // [nsamp][nrcv][2][10][2]
   double* prim;     // [nprim][2][10][2]
   uint32_t mask_g, mask_r, mask_s;
-  uint32_t nrcv, nsamp;
+  uint32_t nrcv, nsamp, njobs;
 };
 
 struct SolveParams {
@@ -192,7 +193,29 @@ struct SolveParams {
   uint64_t counter;
   uint32_t slots;
   uint32_t npend;
+  // several suns in one solve: realisations [job_first[j], job_first[j+1])
+  // run under job_suns[j]; without job_suns the one job is the scene's sun
+  const uint64_t* job_first;
+  const ssolh_sun* job_suns;
+  uint32_t njobs;
 };
+
+// Job of realisation id: the last j with job_first[j] <= id (empty jobs are
+// skipped because their successor starts at the same id)
+__device__ __forceinline__ uint32_t job_of(const SolveParams& prm, uint64_t id) {
+  if (prm.njobs <= 1) return 0;
+  uint32_t lo = 0, hi = prm.njobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (prm.job_first[mid] <= id) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ const ssolh_sun& job_sun(const DScene& S, const SolveParams& prm,
+                                                     uint32_t job) {
+  return prm.job_suns ? prm.job_suns[job] : S.sun;
+}
 
 // Work list: slot of entry q (a null list is the identity)
 __device__ __forceinline__ uint32_t slot_of(const uint32_t* list, uint32_t q) {
@@ -218,7 +241,7 @@ __device__ __forceinline__ void atomic_add(double* p, double v) {
 // sharing the first participant's key are reduced across the wave and
 // flushed by one lane; the others use per-lane atomics. Must be called by
 // the whole wave.
-template <int NV>
+template <int NV, int STRIDE = NV>
 __device__ void keyed_add(bool has, uint64_t key, const double* v, double* base) {
   const uint64_t mask = __ballot(has);
   if (!mask) return;
@@ -228,11 +251,11 @@ __device__ void keyed_add(bool has, uint64_t key, const double* v, double* base)
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     const double s = wave_sum(match ? v[i] : 0.0);
-    if ((int)lane_id() == leader) atomic_add(base + lkey * NV + i, s);
+    if ((int)lane_id() == leader) atomic_add(base + lkey * STRIDE + i, s);
   }
   if (has && !match) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) atomic_add(base + key * NV + i, v[i]);
+    for (int i = 0; i < NV; ++i) atomic_add(base + key * STRIDE + i, v[i]);
   }
 }
 
@@ -389,11 +412,11 @@ __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t is
 // the kernel at 128 VGPRs; the few values that spill are cold.
 template <int STACK>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
-k_trace(DScene S, const uint32_t* __restrict__ list, uint32_t count,
+k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
         const RayRec* __restrict__ rays, HitRec* __restrict__ hits) {
   __shared__ uint32_t lds_stack[STACK * BLOCK];
   const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-  if (q >= count) return;
+  if (q >= *count_ptr) return;
 
   const uint32_t slot = slot_of(list, q);
   const RayRec r = rays[slot];
@@ -543,8 +566,10 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   d3 pos = mul(inst.transform, v0 * b0 + v1 * b1 + v2 * b2) + T;
   const d3 ns = normalize(tmul(inst.inv_linear, cross(v2 - v0, v1 - v0)));
 
-  const d3 dir = sample_sun(S.sun, rng);
-  const double wl = sample_wavelength(S.spectra, S.sun.wl_spectrum, rng.next());
+  const uint32_t job = job_of(prm, path);
+  const ssolh_sun& sun = job_sun(S, prm, job);
+  const d3 dir = sample_sun(sun, rng);
+  const double wl = sample_wavelength(S.spectra, sun.wl_spectrum, rng.next());
 
   const uint32_t iss = inst.isshape_first + st.sshape;
   const IssInfo is = S.iss[iss];
@@ -562,12 +587,12 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   uint32_t side = SSOL_FRONT_;
   if (dot(N, dir) >= 0) { side = SSOL_BACK_; N = -N; }
 
-  const d3 sun0 = ld3(S.sun.dir);
+  const d3 sun0 = ld3(sun.dir);
   const double surface_sun_cos = dot(N, dir);
   const double surface_sun0_cos = fabs(dot(sun0, N));
   const double sun0_sun_cos = dot(sun0, dir);
   const double proxy_cos = punched ? fabs(dot(ns, N)) : 1.0;
-  const double w0 = S.sun.dni * S.S_proxy * fabs(surface_sun_cos / (proxy_cos * sun0_sun_cos));
+  const double w0 = sun.dni * S.S_proxy * fabs(surface_sun_cos / (proxy_cos * sun0_sun_cos));
   const double cosf = S.S_proxy / S.S * surface_sun0_cos / proxy_cos;
 
   const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? is.mtl_front : is.mtl_back];
@@ -581,7 +606,7 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   P.pend[slot] = make_double4(0, 0, 0, 0);
   P.meta[slot] = make_uint4((uint32_t)path, (uint32_t)(path >> 32), rng.block, PF_ACTIVE);
   P.cur[slot] = make_uint4(iss, st.prim, side, (uint32_t)inst.sampled);
-  P.misc[slot] = make_uint4(0, med | (med << 16), 0, 64);
+  P.misc[slot] = make_uint4(0, med | (med << 16), 0, 64u | (job << 8));
   for (uint32_t k = 0; k < prm.npend; ++k) {
     P.pkey[(size_t)k * prm.slots + slot] = NONE;
     P.qkey[(size_t)k * prm.slots + slot] = NONE;
@@ -636,6 +661,7 @@ struct Finish {
   double g[6];      // cos, absorbed_by_receivers, shadowed, missing, extinct, other
   double factor;    // survivor factor applied to pending entries
   int samp_slot;
+  uint32_t job;     // sun of the realisation
 };
 
 // Add the 10 receiver weights of a hit to the per-path pending entries;
@@ -669,8 +695,11 @@ __device__ void spill(double* base, uint64_t key, const double* w, unsigned long
 
 __global__ void __launch_bounds__(BLOCK)
 k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __restrict__ list,
-        uint32_t count, RayRec* __restrict__ rays, const HitRec* __restrict__ hits,
-        uint8_t* __restrict__ emitted, uint32_t* __restrict__ block_emits, Counters* C) {
+        const unsigned* __restrict__ count_ptr, RayRec* __restrict__ rays,
+        const HitRec* __restrict__ hits, uint8_t* __restrict__ emitted,
+        uint32_t* __restrict__ block_emits, Counters* C) {
+  const uint32_t count = *count_ptr;
+  if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: block_count syncs
   const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
   const bool live = q < count;
   const size_t stride = prm.slots;
@@ -681,6 +710,7 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
   fin.kind = FIN_NONE;
   fin.factor = 1;
   fin.samp_slot = 0;
+  fin.job = 0;
   for (int i = 0; i < 6; ++i) fin.g[i] = 0;
   const uint32_t slot = live ? slot_of(list, q) : 0;
 
@@ -703,7 +733,8 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
     fin.samp_slot = (int)cur.w;
 
     uint32_t in_med = misc.y & 0xFFFFu, out_med = misc.y >> 16;
-    uint32_t depth = misc.x, score = misc.z, rinterval = misc.w;
+    uint32_t depth = misc.x, score = misc.z, rinterval = misc.w & 0xFFu;
+    fin.job = misc.w >> 8;
     d3 pos = mk(pw.x, pw.y, pw.z);
     const double wl = pw.w;
     d3 dir = mk(dc.x, dc.y, dc.z);
@@ -884,7 +915,7 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
       P.pend[slot] = pd;
       P.meta[slot] = make_uint4(meta.x, meta.y, rng.block, flags);
       P.cur[slot] = make_uint4(iss, prim, side, cur.w);
-      P.misc[slot] = make_uint4(depth, in_med | (out_med << 16), score, rinterval);
+      P.misc[slot] = make_uint4(depth, in_med | (out_med << 16), score, rinterval | (fin.job << 8));
     }
   }
 
@@ -892,18 +923,19 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
   const bool done = fin.kind == FIN_DONE;
   double* rcv_base = T.rcv + (size_t)(blockIdx.x & T.mask_r) * T.nrcv * 40u;
   {
-    const uint32_t sg = blockIdx.x & T.mask_g;
-    double g2[12];
+    // global fields and the realisation count of the path's job (sun)
+    const bool counted = done || fin.kind == FIN_KILLED;
+    double g2[13];
     for (int i = 0; i < 6; ++i) {
       g2[2 * i] = done ? fin.g[i] : 0.0;
       g2[2 * i + 1] = done ? fin.g[i] * fin.g[i] : 0.0;
     }
-    keyed_add<12>(done, 0, g2, T.global + (size_t)sg * 16u);
+    g2[12] = 1.0;
+    keyed_add<13, 16>(counted, counted ? fin.job : 0, g2,
+                      T.global + (size_t)(blockIdx.x & T.mask_g) * T.njobs * 16u);
     double s4[4] = {fin.g[0], fin.g[0] * fin.g[0], fin.g[2], fin.g[2] * fin.g[2]};
     keyed_add<4>(done, (uint64_t)fin.samp_slot, s4,
                  T.samp + (size_t)(blockIdx.x & T.mask_s) * T.nsamp * 4u);
-    const uint64_t m = __ballot(done || fin.kind == FIN_KILLED);
-    if (lane_id() == 0 && m) atomicAdd(T.realisations + (size_t)sg * 16u, (unsigned long long)__popcll(m));
     const unsigned long long nf = wave_count_add(fin.kind == FIN_FAILED, &C->failed);
     if (fin.kind == FIN_FAILED && nf + 1 >= prm.max_failed) atomicExch(&C->aborted, 1u);
   }
@@ -972,10 +1004,12 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
 // Exclusive scan of the per-block emit counts of k_shade, by one block;
 // the grand total is the next list's length.
 __global__ void __launch_bounds__(SCAN_THREADS)
-k_scan(const uint32_t* __restrict__ counts, uint32_t nblocks, uint32_t* __restrict__ offsets,
-       unsigned int* total) {
+k_scan(const uint32_t* __restrict__ counts, const unsigned* __restrict__ count_ptr,
+       uint32_t* __restrict__ offsets, unsigned int* total, unsigned long long* segments) {
   __shared__ uint32_t part[SCAN_THREADS];
   const uint32_t tid = threadIdx.x;
+  const uint32_t nblocks = (*count_ptr + BLOCK - 1) / BLOCK;
+  if (tid == 0) *segments += *count_ptr;
   const uint32_t per = (nblocks + SCAN_THREADS - 1) / SCAN_THREADS;
   const uint32_t b0 = min(tid * per, nblocks), b1 = min(b0 + per, nblocks);
   uint32_t s = 0;
@@ -999,9 +1033,12 @@ k_scan(const uint32_t* __restrict__ counts, uint32_t nblocks, uint32_t* __restri
 // Entry q of the next list: the q-th slot (in list order) that wrote a ray.
 // Same block partition as k_shade.
 __global__ void __launch_bounds__(BLOCK)
-k_scatter(const uint32_t* __restrict__ list, uint32_t count, const uint8_t* __restrict__ emitted,
-          const uint32_t* __restrict__ offsets, uint32_t* __restrict__ next) {
+k_scatter(const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
+          const uint8_t* __restrict__ emitted, const uint32_t* __restrict__ offsets,
+          uint32_t* __restrict__ next) {
   __shared__ uint32_t wcount[WAVES];
+  const uint32_t count = *count_ptr;
+  if (blockIdx.x * BLOCK >= count) return;  // whole blocks only
   const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
   const bool live = q < count;
   const uint32_t slot = live ? slot_of(list, q) : 0;
@@ -1017,6 +1054,13 @@ k_scatter(const uint32_t* __restrict__ list, uint32_t count, const uint8_t* __re
 }
 
 #include "ssolh_fp32.hpp"
+
+// First work list: slots [0, n) in order, and its length
+__global__ void k_iota(uint32_t* __restrict__ list, uint32_t n, unsigned* __restrict__ count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) list[i] = i;
+  if (i == 0) *count = n;
+}
 
 // Sum of the stripes of a striped tally into stripe 0
 template <typename T>
@@ -1115,7 +1159,8 @@ struct ssolh_context {
   uint64_t nprim = 0;
   uint32_t stack_depth = MAX_STACK;
   // solve buffers
-  DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, counters, tallies;
+  DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, counters, tallies, jobs;
+  uint32_t njobs = 1;
   PathState P{};
   PathState32 P32{};
   const ShapeQ* shapeq = nullptr;  // per shape, in scene_bufs
@@ -1123,7 +1168,6 @@ struct ssolh_context {
   size_t tally_bytes = 0;
   // last solve
   Counters last{};
-  unsigned long long realisations = 0;
   uint64_t segments = 0;
   double kernel_ms = 0, trace_ms = 0, shade_ms = 0;
   uint32_t iterations = 0;
@@ -1245,7 +1289,7 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_scene(c);
   for (DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
-                    &c->block_emits, &c->block_offsets, &c->counters, &c->tallies})
+                    &c->block_emits, &c->block_offsets, &c->counters, &c->tallies, &c->jobs})
     if (b->p) (void)hipFree(b->p);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1322,19 +1366,20 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   return 0;
 }
 
-SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
-  if (!c || !a || !a->realisations) { set_err("ssolh_solve: invalid arguments"); return SSOLH_ERR_ARG; }
-  if (!c->has_scene) { set_err("ssolh_solve: no scene uploaded"); return SSOLH_ERR_ARG; }
-  if (a->precision != SSOLH_FP64 && a->precision != SSOLH_FP32) {
-    set_err("ssolh_solve: unknown precision %u", a->precision);
-    return SSOLH_ERR_ARG;
-  }
-  const bool fp32 = a->precision == SSOLH_FP32;
-  HIP_TRY(hipSetDevice(c->device));
-  const uint32_t npend = a->pending_entries ? a->pending_entries : 4;
+}  // extern "C"
+
+namespace {
+
+// One solve over `njobs` suns (njobs == 1: the scene's sun). Realisation
+// ids [0, N) map to jobs through job_first. graph_iters > 0 captures that
+// many iterations (rounded up to even) into a HIP graph replayed until the
+// work list is empty; 0 runs the iterations from the host one by one.
+int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, uint64_t counter,
+              bool fp32, uint32_t path_slots, uint32_t npend, uint32_t njobs,
+              const uint64_t* job_first_h, const ssolh_sun* job_suns_h, uint32_t graph_iters) {
   if (npend > MAX_PEND) { set_err("ssolh_solve: at most %d pending entries", MAX_PEND); return SSOLH_ERR_LIMIT; }
-  uint64_t slots64 = a->path_slots ? a->path_slots : (1u << 21);
-  if (slots64 > a->realisations) slots64 = a->realisations;
+  uint64_t slots64 = path_slots ? path_slots : (1u << 21);
+  if (slots64 > N) slots64 = N;
   slots64 = (slots64 + BLOCK - 1) / BLOCK * BLOCK;
   if (slots64 > (1ull << 28)) { set_err("ssolh_solve: at most 2^28 path slots"); return SSOLH_ERR_LIMIT; }
   const uint32_t slots = (uint32_t)slots64;
@@ -1390,7 +1435,15 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
   if ((r = dev_alloc(c->emitted, n))) return r;
   if ((r = dev_alloc(c->block_emits, max_blocks * sizeof(uint32_t)))) return r;
   if ((r = dev_alloc(c->block_offsets, max_blocks * sizeof(uint32_t)))) return r;
-  if ((r = dev_alloc(c->counters, sizeof(Counters)))) return r;
+  if ((r = dev_alloc(c->counters, sizeof(Counters) + 4 * sizeof(unsigned)))) return r;
+  // jobs
+  if (job_suns_h) {
+    if ((r = dev_alloc(c->jobs, (njobs + 1) * sizeof(uint64_t) + njobs * sizeof(ssolh_sun)))) return r;
+    HIP_TRY(hipMemcpyAsync(c->jobs.p, job_first_h, (njobs + 1) * sizeof(uint64_t),
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync((char*)c->jobs.p + (njobs + 1) * sizeof(uint64_t), job_suns_h,
+                           njobs * sizeof(ssolh_sun), hipMemcpyHostToDevice, c->stream));
+  }
 
   // tallies (striped where many waves meet, see Tallies)
   auto pow2_stripes = [](size_t per_stripe_doubles) {
@@ -1398,12 +1451,12 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
     while (k > 1 && k * per_stripe_doubles > (size_t(1) << 21)) k >>= 1;
     return (uint32_t)k;
   };
-  const uint32_t sg = 256, sr = pow2_stripes((size_t)c->nrcv * 40),
+  const uint32_t sg = pow2_stripes((size_t)njobs * 16), sr = pow2_stripes((size_t)c->nrcv * 40),
                  ss = pow2_stripes((size_t)c->nsamp * 5);
-  const size_t n_global = (size_t)sg * 16, n_rcv = (size_t)sr * c->nrcv * 40,
+  const size_t n_global = (size_t)sg * njobs * 16, n_rcv = (size_t)sr * c->nrcv * 40,
                n_samp = (size_t)ss * c->nsamp * 4;
   const size_t n_sxr = (size_t)c->nsamp * c->nrcv * 40, n_prim = (size_t)c->nprim * 40;
-  const size_t n_u64 = (size_t)sg * 16 + (size_t)ss * c->nsamp;
+  const size_t n_u64 = (size_t)ss * c->nsamp;
   const size_t tally_bytes = (n_global + n_rcv + n_samp + n_sxr + n_prim + n_u64) * 8 + 64;
   if ((r = dev_alloc(c->tallies, tally_bytes))) return r;
   c->tally_bytes = tally_bytes;
@@ -1413,54 +1466,40 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
   c->T.samp = c->T.rcv + n_rcv;
   c->T.sxr = c->T.samp + n_samp;
   c->T.prim = c->T.sxr + n_sxr;
-  c->T.realisations = (unsigned long long*)(c->T.prim + n_prim);
-  c->T.samp_count = c->T.realisations + (size_t)sg * 16;
+  c->T.samp_count = (unsigned long long*)(c->T.prim + n_prim);
   c->T.mask_g = sg - 1;
   c->T.mask_r = sr - 1;
   c->T.mask_s = ss - 1;
   c->T.nrcv = c->nrcv;
   c->T.nsamp = c->nsamp;
+  c->T.njobs = njobs;
+  c->njobs = njobs;
   HIP_TRY(hipMemsetAsync(c->tallies.p, 0, tally_bytes, c->stream));
-  HIP_TRY(hipMemsetAsync(c->counters.p, 0, sizeof(Counters), c->stream));
+  HIP_TRY(hipMemsetAsync(c->counters.p, 0, sizeof(Counters) + 4 * sizeof(unsigned), c->stream));
 
   SolveParams prm;
-  prm.N = a->realisations;
-  prm.max_failed = a->max_failed;
-  prm.key = a->rng_key;
-  prm.counter = a->rng_counter;
+  prm.N = N;
+  prm.max_failed = max_failed;
+  prm.key = key;
+  prm.counter = counter;
   prm.slots = slots;
   prm.npend = npend;
+  prm.njobs = njobs;
+  prm.job_first = job_suns_h ? (const uint64_t*)c->jobs.p : nullptr;
+  prm.job_suns = job_suns_h ? (const ssolh_sun*)((char*)c->jobs.p + (njobs + 1) * sizeof(uint64_t))
+                            : nullptr;
 
   Counters* ctr = (Counters*)c->counters.p;
-  RayRec* rays = (RayRec*)c->rays.p;
-  HitRec* hits = (HitRec*)c->hits.p;
+  unsigned* cnt = (unsigned*)((char*)c->counters.p + sizeof(Counters));  // [2] list lengths
   uint32_t* lists[2] = {(uint32_t*)c->lists[0].p, (uint32_t*)c->lists[1].p};
   uint8_t* emitted = (uint8_t*)c->emitted.p;
   uint32_t* block_emits = (uint32_t*)c->block_emits.p;
   uint32_t* block_offsets = (uint32_t*)c->block_offsets.p;
 
-  float ms_trace = 0, ms_shade = 0;
-  HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  // the first realisations fill slots [0, count): the list is the identity
-  uint32_t count = (uint32_t)std::min<uint64_t>(a->realisations, slots);
-  if (fp32)
-    k_generate32<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
-        c->scn, c->shapeq, c->P32, prm, c->T, (RayRec32*)c->rays.p, count);
-  else
-    k_generate<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, rays,
-                                                                     count);
-  HIP_TRY(hipGetLastError());
-  const uint32_t* list = nullptr;
-  int next = 0;
-  uint32_t iter = 0;
-  uint64_t segments = 0;  // one per traced ray
-  while (count) {
-    if (iter >= 1000000u) {
-      set_err("ssolh_solve: the wavefront did not drain after %u iterations", iter);
-      return SSOLH_ERR_LIMIT;
-    }
-    const uint32_t blocks = (count + BLOCK - 1) / BLOCK;
-    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+  // one iteration: trace, shade, compaction of list i into list i^1
+  auto iteration = [&](int i, uint32_t blocks) -> int {
+    const uint32_t* list = lists[i];
+    const unsigned* count = cnt + i;
     if (fp32) {
       RayRec32* r32 = (RayRec32*)c->rays.p;
       HitRec32* h32 = (HitRec32*)c->hits.p;
@@ -1473,58 +1512,123 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
       else
         k_trace32<MAX_STACK><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list, count, r32, h32);
     } else {
+      RayRec* r64 = (RayRec*)c->rays.p;
+      HitRec* h64 = (HitRec*)c->hits.p;
       if (c->stack_depth <= 16)
-        k_trace<16><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+        k_trace<16><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, r64, h64);
       else if (c->stack_depth <= 24)
-        k_trace<24><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+        k_trace<24><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, r64, h64);
       else if (c->stack_depth <= 32)
-        k_trace<32><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+        k_trace<32><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, r64, h64);
       else
-        k_trace<MAX_STACK><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, rays, hits);
+        k_trace<MAX_STACK><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, r64, h64);
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    if (!graph_iters) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     if (fp32)
       k_shade32<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, c->P32, prm, c->T, list, count,
                                                  (RayRec32*)c->rays.p, (const HitRec32*)c->hits.p,
                                                  emitted, block_emits, ctr);
     else
-      k_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count, rays, hits,
+      k_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
+                                               (RayRec*)c->rays.p, (const HitRec*)c->hits.p,
                                                emitted, block_emits, ctr);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-    k_scan<<<1, SCAN_THREADS, 0, c->stream>>>(block_emits, blocks, block_offsets, &ctr->total);
+    if (!graph_iters) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+    k_scan<<<1, SCAN_THREADS, 0, c->stream>>>(block_emits, count, block_offsets, cnt + (i ^ 1),
+                                              &ctr->segments);
     HIP_TRY(hipGetLastError());
-    k_scatter<<<blocks, BLOCK, 0, c->stream>>>(list, count, emitted, block_offsets, lists[next]);
+    k_scatter<<<blocks, BLOCK, 0, c->stream>>>(list, count, emitted, block_offsets, lists[i ^ 1]);
     HIP_TRY(hipGetLastError());
-    unsigned int total = 0;
-    HIP_TRY(hipMemcpyAsync(&total, &ctr->total, sizeof(total), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    float t1 = 0, t2 = 0;
-    (void)hipEventElapsedTime(&t1, c->ev[1], c->ev[2]);
-    (void)hipEventElapsedTime(&t2, c->ev[2], c->ev[3]);
-    ms_trace += t1;
-    ms_shade += t2;
-    if (total > count) {
-      set_err("ssolh_solve: work list grew from %u to %u entries", count, total);
-      return SSOLH_ERR_LIMIT;
+    return 0;
+  };
+
+  float ms_trace = 0, ms_shade = 0;
+  HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  // the first realisations fill slots [0, count): list 0 is 0..count-1
+  uint32_t count = (uint32_t)std::min<uint64_t>(N, slots);
+  k_iota<<<(count + 255) / 256, 256, 0, c->stream>>>(lists[0], count, cnt);
+  HIP_TRY(hipGetLastError());
+  if (fp32)
+    k_generate32<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
+        c->scn, c->shapeq, c->P32, prm, c->T, (RayRec32*)c->rays.p, count);
+  else
+    k_generate<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
+        c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count);
+  HIP_TRY(hipGetLastError());
+  uint32_t iter = 0;
+  if (!graph_iters) {
+    int cur = 0;
+    while (count) {
+      if (iter >= 1000000u) {
+        set_err("ssolh_solve: the wavefront did not drain after %u iterations", iter);
+        return SSOLH_ERR_LIMIT;
+      }
+      HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+      if ((r = iteration(cur, (count + BLOCK - 1) / BLOCK))) return r;
+      unsigned total = 0;
+      HIP_TRY(hipMemcpyAsync(&total, cnt + (cur ^ 1), sizeof(total), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      float t1 = 0, t2 = 0;
+      (void)hipEventElapsedTime(&t1, c->ev[1], c->ev[2]);
+      (void)hipEventElapsedTime(&t2, c->ev[2], c->ev[3]);
+      ms_trace += t1;
+      ms_shade += t2;
+      if (total > count) {
+        set_err("ssolh_solve: work list grew from %u to %u entries", count, total);
+        return SSOLH_ERR_LIMIT;
+      }
+      cur ^= 1;
+      count = total;
+      ++iter;
     }
-    list = lists[next];
-    next ^= 1;
-    segments += count;
-    count = total;
-    ++iter;
+  } else {
+    // graph of an even number of iterations: list 0 -> 1 -> 0 ...; every
+    // kernel reads its work count from device memory, so iterations past
+    // the end of the work are empty launches
+    const uint32_t gi = (graph_iters + 1) & ~1u;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int cr = 0;
+    for (uint32_t k = 0; k < gi && !cr; ++k) cr = iteration((int)(k & 1), max_blocks);
+    const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+    if (cr) { if (graph) (void)hipGraphDestroy(graph); return cr; }
+    HIP_TRY(ce);
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      HIP_TRY(ie);
+    }
+    int gerr = 0;
+    while (count) {
+      if (iter >= 1000000u) {
+        set_err("ssolh_solve: the wavefront did not drain after %u iterations", iter);
+        gerr = SSOLH_ERR_LIMIT;
+        break;
+      }
+      const hipError_t le = hipGraphLaunch(exec, c->stream);
+      if (le != hipSuccess) { set_err("hipGraphLaunch: %s", hipGetErrorString(le)); gerr = -(int)le; break; }
+      unsigned total = 0;
+      const hipError_t me = hipMemcpyAsync(&total, cnt, sizeof(total), hipMemcpyDeviceToHost, c->stream);
+      const hipError_t se = me == hipSuccess ? hipStreamSynchronize(c->stream) : me;
+      if (se != hipSuccess) { set_err("graph solve: %s", hipGetErrorString(se)); gerr = -(int)se; break; }
+      count = total;
+      iter += gi;
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (gerr) return gerr;
   }
   // fold the stripes
-  auto fold = [&](auto* base, size_t n, uint32_t stripes) -> int {
-    if (stripes > 1 && n) {
-      k_fold<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(base, n, stripes);
+  auto fold = [&](auto* base, size_t nn, uint32_t stripes) -> int {
+    if (stripes > 1 && nn) {
+      k_fold<<<(unsigned)((nn + 255) / 256), 256, 0, c->stream>>>(base, nn, stripes);
       HIP_TRY(hipGetLastError());
     }
     return 0;
   };
-  if ((r = fold(c->T.global, 16, sg))) return r;
-  if ((r = fold(c->T.realisations, 16, sg))) return r;
+  if ((r = fold(c->T.global, (size_t)njobs * 16, sg))) return r;
   if ((r = fold(c->T.rcv, (size_t)c->nrcv * 40, sr))) return r;
   if ((r = fold(c->T.samp, (size_t)c->nsamp * 4, ss))) return r;
   if ((r = fold(c->T.samp_count, (size_t)c->nsamp, ss))) return r;
@@ -1533,13 +1637,61 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   (void)hipEventElapsedTime(&total_ms, c->ev[0], c->ev[3]);
   HIP_TRY(hipMemcpy(&c->last, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(&c->realisations, c->T.realisations, sizeof(c->realisations),
-                    hipMemcpyDeviceToHost));
-  c->segments = segments;
+  c->segments = c->last.segments;
   c->kernel_ms = total_ms;
   c->trace_ms = ms_trace;
   c->shade_ms = ms_shade;
   c->iterations = iter;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
+  if (!c || !a || !a->realisations) { set_err("ssolh_solve: invalid arguments"); return SSOLH_ERR_ARG; }
+  if (!c->has_scene) { set_err("ssolh_solve: no scene uploaded"); return SSOLH_ERR_ARG; }
+  if (a->precision != SSOLH_FP64 && a->precision != SSOLH_FP32) {
+    set_err("ssolh_solve: unknown precision %u", a->precision);
+    return SSOLH_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  const uint32_t npend = a->pending_entries ? a->pending_entries : 4;
+  return run_solve(c, a->realisations, a->max_failed, a->rng_key, a->rng_counter,
+                   a->precision == SSOLH_FP32, a->path_slots, npend, 1, nullptr, nullptr,
+                   a->graph_iterations);
+}
+
+SSOLH_API int ssolh_solve_batch(ssolh_context* c, const struct ssolh_batch_args* a,
+                                struct ssolh_batch_result* out) {
+  if (!c || !a || !out || !a->nsuns || !a->suns || !a->realisations) {
+    set_err("ssolh_solve_batch: invalid arguments");
+    return SSOLH_ERR_ARG;
+  }
+  if (!c->has_scene) { set_err("ssolh_solve_batch: no scene uploaded"); return SSOLH_ERR_ARG; }
+  if (a->precision != SSOLH_FP64 && a->precision != SSOLH_FP32) {
+    set_err("ssolh_solve_batch: unknown precision %u", a->precision);
+    return SSOLH_ERR_ARG;
+  }
+  if (a->nsuns > (1u << 24)) { set_err("ssolh_solve_batch: at most 2^24 suns"); return SSOLH_ERR_LIMIT; }
+  std::vector<uint64_t> first(a->nsuns + 1, 0);
+  for (uint32_t j = 0; j < a->nsuns; ++j) first[j + 1] = first[j] + a->realisations[j];
+  if (!first[a->nsuns]) { set_err("ssolh_solve_batch: no realisation"); return SSOLH_ERR_ARG; }
+  HIP_TRY(hipSetDevice(c->device));
+  const uint32_t npend = a->pending_entries ? a->pending_entries : 4;
+  const uint32_t njobs = a->nsuns;
+  std::vector<ssolh_sun> suns(a->suns, a->suns + a->nsuns);
+  int r = run_solve(c, first[a->nsuns], a->max_failed, a->rng_key, a->rng_counter,
+                    a->precision == SSOLH_FP32, a->path_slots, npend, njobs, first.data(),
+                    suns.data(), a->graph_iterations);
+  if (r) return r;
+  std::vector<double> g((size_t)njobs * 16);
+  HIP_TRY(hipMemcpy(g.data(), c->T.global, g.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (uint32_t j = 0; j < njobs; ++j) {
+    memcpy(out[j].global, &g[(size_t)j * 16], 12 * sizeof(double));
+    out[j].realisation_count = (uint64_t)g[(size_t)j * 16 + 12];
+  }
   return 0;
 }
 
@@ -1556,7 +1708,9 @@ SSOLH_API int ssolh_tallies_download(ssolh_context* c, struct ssolh_tallies* o) 
   if (o->sampled_x_receivers && n_sxr)
     HIP_TRY(hipMemcpy(o->sampled_x_receivers, c->T.sxr, n_sxr * 8, hipMemcpyDeviceToHost));
   if (o->primitives && n_prim) HIP_TRY(hipMemcpy(o->primitives, c->T.prim, n_prim * 8, hipMemcpyDeviceToHost));
-  o->realisation_count = c->realisations;
+  double cnt0 = 0;
+  HIP_TRY(hipMemcpy(&cnt0, c->T.global + 12, sizeof(double), hipMemcpyDeviceToHost));
+  o->realisation_count = (uint64_t)cnt0;
   o->failed_count = c->last.failed;
   o->segments = c->segments;
   o->pending_spills = c->last.spills;

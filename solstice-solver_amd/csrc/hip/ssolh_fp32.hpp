@@ -167,10 +167,11 @@ __device__ bool filter32(const DScene& S, const ShapeQ* SQ, const RayRec32& r, f
 template <int STACK>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_trace32(DScene S, const ShapeQ* __restrict__ SQ, const uint32_t* __restrict__ list,
-          uint32_t count, const RayRec32* __restrict__ rays, HitRec32* __restrict__ hits) {
+          const unsigned* __restrict__ count_ptr, const RayRec32* __restrict__ rays,
+          HitRec32* __restrict__ hits) {
   __shared__ uint32_t lds_stack[STACK * BLOCK];
   const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-  if (q >= count) return;
+  if (q >= *count_ptr) return;
 
   const uint32_t slot = slot_of(list, q);
   const RayRec32 r = rays[slot];
@@ -334,8 +335,10 @@ __device__ void generate32(const DScene& S, const ShapeQ* SQ, const PathState32&
   f3 off = mulf(inst.transform, p_obj);
   const f3 ns = normalize(tmulf(inst.inv_linear, cross(v2 - v0, v1 - v0)));
 
-  const f3 dir = sample_sun32(S.sun, rng);
-  const float wl = (float)sample_wavelength(S.spectra, S.sun.wl_spectrum, (double)rng.next());
+  const uint32_t job = job_of(prm, path);
+  const ssolh_sun& sun = job_sun(S, prm, job);
+  const f3 dir = sample_sun32(sun, rng);
+  const float wl = (float)sample_wavelength(S.spectra, sun.wl_spectrum, (double)rng.next());
 
   const uint32_t iss = inst.isshape_first + st.sshape;
   const IssInfo is = S.iss[iss];
@@ -355,12 +358,12 @@ __device__ void generate32(const DScene& S, const ShapeQ* SQ, const PathState32&
   uint32_t side = SSOL_FRONT_;
   if (dot(N, dir) >= 0) { side = SSOL_BACK_; N = -N; }
 
-  const f3 sun0 = mkf((float)S.sun.dir[0], (float)S.sun.dir[1], (float)S.sun.dir[2]);
+  const f3 sun0 = mkf((float)sun.dir[0], (float)sun.dir[1], (float)sun.dir[2]);
   const float surface_sun_cos = dot(N, dir);
   const float surface_sun0_cos = fabsf(dot(sun0, N));
   const float sun0_sun_cos = dot(sun0, dir);
   const float proxy_cos = punched ? fabsf(dot(ns, N)) : 1.0f;
-  const float w0 = (float)(S.sun.dni * S.S_proxy) * fabsf(surface_sun_cos / (proxy_cos * sun0_sun_cos));
+  const float w0 = (float)(sun.dni * S.S_proxy) * fabsf(surface_sun_cos / (proxy_cos * sun0_sun_cos));
   const float cosf_ = (float)(S.S_proxy / S.S) * surface_sun0_cos / proxy_cos;
 
   const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? is.mtl_front : is.mtl_back];
@@ -374,7 +377,7 @@ __device__ void generate32(const DScene& S, const ShapeQ* SQ, const PathState32&
   P.pend_cos[slot] = make_float4(0, 0, 0, cosf_);
   P.meta[slot] = make_uint4((uint32_t)path, (uint32_t)(path >> 32), rng.block, PF_ACTIVE);
   P.cur[slot] = make_uint4(iss, st.prim, side, (uint32_t)inst.sampled);
-  P.misc[slot] = make_uint4(0, med | (med << 16), 0, 64);
+  P.misc[slot] = make_uint4(0, med | (med << 16), 0, 64u | (job << 8));
   for (uint32_t k = 0; k < prm.npend; ++k) {
     P.pkey[(size_t)k * prm.slots + slot] = NONE;
     P.qkey[(size_t)k * prm.slots + slot] = NONE;
@@ -436,9 +439,11 @@ __device__ void spill32(double* base, uint64_t key, const float* w, unsigned lon
 // src/ssol_solver.c:939-1086); same structure as k_shade.
 __global__ void __launch_bounds__(BLOCK)
 k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams prm, Tallies T,
-          const uint32_t* __restrict__ list, uint32_t count, RayRec32* __restrict__ rays,
-          const HitRec32* __restrict__ hits, uint8_t* __restrict__ emitted,
-          uint32_t* __restrict__ block_emits, Counters* C) {
+          const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
+          RayRec32* __restrict__ rays, const HitRec32* __restrict__ hits,
+          uint8_t* __restrict__ emitted, uint32_t* __restrict__ block_emits, Counters* C) {
+  const uint32_t count = *count_ptr;
+  if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: block_count syncs
   const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
   const bool live = q < count;
   const size_t stride = prm.slots;
@@ -449,6 +454,7 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
   fin.kind = FIN_NONE;
   fin.factor = 1;
   fin.samp_slot = 0;
+  fin.job = 0;
   for (int i = 0; i < 6; ++i) fin.g[i] = 0;
   const uint32_t slot = live ? slot_of(list, q) : 0;
 
@@ -471,7 +477,8 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
     fin.samp_slot = (int)cur.w;
 
     uint32_t in_med = misc.y & 0xFFFFu, out_med = misc.y >> 16;
-    uint32_t depth = misc.x, score = misc.z, rinterval = misc.w;
+    uint32_t depth = misc.x, score = misc.z, rinterval = misc.w & 0xFFu;
+    fin.job = misc.w >> 8;
     f3 pos = mkf(ps.x, ps.y, ps.z);
     uint32_t base = __float_as_uint(ps.w);
     const float wl = dw.w;
@@ -654,25 +661,26 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
       P.pend_cos[slot] = pc;
       P.meta[slot] = make_uint4(meta.x, meta.y, rng.block, flags);
       P.cur[slot] = make_uint4(iss, prim, side, cur.w);
-      P.misc[slot] = make_uint4(depth, in_med | (out_med << 16), score, rinterval);
+      P.misc[slot] = make_uint4(depth, in_med | (out_med << 16), score, rinterval | (fin.job << 8));
     }
   }
 
   const bool done = fin.kind == FIN_DONE;
   double* rcv_base = T.rcv + (size_t)(blockIdx.x & T.mask_r) * T.nrcv * 40u;
   {
-    const uint32_t sg = blockIdx.x & T.mask_g;
-    double g2[12];
+    // global fields and the realisation count of the path's job (sun)
+    const bool counted = done || fin.kind == FIN_KILLED;
+    double g2[13];
     for (int i = 0; i < 6; ++i) {
       g2[2 * i] = done ? fin.g[i] : 0.0;
       g2[2 * i + 1] = done ? fin.g[i] * fin.g[i] : 0.0;
     }
-    keyed_add<12>(done, 0, g2, T.global + (size_t)sg * 16u);
+    g2[12] = 1.0;
+    keyed_add<13, 16>(counted, counted ? fin.job : 0, g2,
+                      T.global + (size_t)(blockIdx.x & T.mask_g) * T.njobs * 16u);
     double s4[4] = {fin.g[0], fin.g[0] * fin.g[0], fin.g[2], fin.g[2] * fin.g[2]};
     keyed_add<4>(done, (uint64_t)fin.samp_slot, s4,
                  T.samp + (size_t)(blockIdx.x & T.mask_s) * T.nsamp * 4u);
-    const uint64_t m = __ballot(done || fin.kind == FIN_KILLED);
-    if (lane_id() == 0 && m) atomicAdd(T.realisations + (size_t)sg * 16u, (unsigned long long)__popcll(m));
     const unsigned long long nf = wave_count_add(fin.kind == FIN_FAILED, &C->failed);
     if (fin.kind == FIN_FAILED && nf + 1 >= prm.max_failed) atomicExch(&C->aborted, 1u);
   }

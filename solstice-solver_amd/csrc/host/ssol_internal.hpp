@@ -34,6 +34,7 @@ struct ssol_device : ssol_refcounted {
   unsigned nthreads = 1;
   enum ssol_precision precision = SSOL_PRECISION_FP64;
   size_t path_slots = 0;
+  unsigned graph_iterations = 0;  // 0: host-driven wavefront iterations
   ssolh_context* gpu = nullptr;  // created at the first solve
   solve_cache* cache = nullptr;
 };

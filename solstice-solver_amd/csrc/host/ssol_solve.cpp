@@ -7,6 +7,7 @@
 // layout), and runs the realisations on the GPU through ssolh_*. There is no
 // CPU path. The estimator getters restate src/ssol_estimator.c:112-268 and
 // src/ssol_mc_receiver.c:32-173 over the downloaded tally arrays.
+#include <chrono>
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -427,6 +428,45 @@ std::vector<uint64_t> scene_signature(const ssol_scene* scn) {
   return sig;
 }
 
+// Device form of a sun (src/ssol_sun.c:214-279). With add_spectrum false
+// the sun's spectrum must already be in the flat scene's pool.
+res_T flat_sun(Flat& f, const ssol_sun* sun, bool add_spectrum, ssolh_sun& su) {
+  memset(&su, 0, sizeof(su));
+  if (!sun->spectrum || sun->spectrum->wavelengths.empty() || !(sun->dni > 0)) return RES_BAD_ARG;
+  for (int k = 0; k < 3; ++k) su.dir[k] = sun->direction[k];
+  if (su.dir[0] == 0 && su.dir[1] == 0 && su.dir[2] == 0) return RES_BAD_ARG;
+  frame(su.dir, su.basis);
+  su.dni = sun->dni;
+  if (add_spectrum) {
+    su.wl_spectrum = spectrum_id(f, sun->spectrum);
+  } else {
+    auto it = f.spec_index.find(sun->spectrum);
+    if (it == f.spec_index.end()) return RES_BAD_ARG;
+    su.wl_spectrum = it->second;
+  }
+  switch (sun->kind) {
+    case sun_kind::directional: su.type = SSOLH_SUN_DIRECTIONAL; break;
+    case sun_kind::pillbox:
+      if (!(sun->param > 0)) return RES_BAD_ARG;
+      su.type = SSOLH_SUN_PILLBOX;
+      su.param[0] = std::sin(sun->param) * std::sin(sun->param);
+      break;
+    case sun_kind::gaussian:
+      if (!(sun->param > 0)) return RES_BAD_ARG;
+      su.type = SSOLH_SUN_GAUSSIAN;
+      su.param[0] = sun->param;
+      break;
+    case sun_kind::buie:
+      if (sun->param < 1e-6 || sun->param > 0.849) return RES_BAD_ARG;
+      su.type = SSOLH_SUN_BUIE;
+      buie_params(sun->param, su.param);
+      break;
+  }
+  const ssolh_spectrum& sp = f.spectra[su.wl_spectrum];
+  su.wl_cdf_total = sp.count ? f.cdf[sp.offset + sp.count - 1] : 0;
+  return RES_OK;
+}
+
 res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
   ssol_device* dev = scn->dev;
   ssol_sun* sun = scn->sun;
@@ -563,33 +603,9 @@ res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
               f.instances.size(), f.tris.size(), tdepth, max_blas);
 
   // sun (src/ssol_sun.c:214-279)
-  ssolh_sun& su = f.scn.sun;
-  memset(&su, 0, sizeof(su));
-  for (int k = 0; k < 3; ++k) su.dir[k] = sun->direction[k];
-  frame(su.dir, su.basis);
-  su.dni = sun->dni;
-  su.wl_spectrum = spectrum_id(f, sun->spectrum);
-  switch (sun->kind) {
-    case sun_kind::directional: su.type = SSOLH_SUN_DIRECTIONAL; break;
-    case sun_kind::pillbox:
-      if (!(sun->param > 0)) return RES_BAD_ARG;
-      su.type = SSOLH_SUN_PILLBOX;
-      su.param[0] = std::sin(sun->param) * std::sin(sun->param);
-      break;
-    case sun_kind::gaussian:
-      if (!(sun->param > 0)) return RES_BAD_ARG;
-      su.type = SSOLH_SUN_GAUSSIAN;
-      su.param[0] = sun->param;
-      break;
-    case sun_kind::buie:
-      if (sun->param < 1e-6 || sun->param > 0.849) return RES_BAD_ARG;
-      su.type = SSOLH_SUN_BUIE;
-      buie_params(sun->param, su.param);
-      break;
-  }
   {
-    const ssolh_spectrum& sp = f.spectra[su.wl_spectrum];
-    su.wl_cdf_total = sp.count ? f.cdf[sp.offset + sp.count - 1] : 0;
+    const res_T rs = flat_sun(f, sun, true, f.scn.sun);
+    if (rs != RES_OK) return rs;
   }
 
   ssolh_scene& s = f.scn;
@@ -668,25 +684,13 @@ void solve_cache_free(ssol_device* d) {
 }
 }  // namespace ssol
 
-extern "C" {
+namespace {
 
-res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t N,
-                 const size_t max_failed, const struct ssol_path_tracker* tracker,
-                 struct ssol_estimator** out) {
-  if (!scn || !rng || !N || !out) return RES_BAD_ARG;
-  *out = nullptr;
-  if (N > (size_t)INT64_MAX || max_failed > (size_t)INT64_MAX) return RES_BAD_ARG;
+// Flattened scene of `scn` on the device: reuses the cached copy when no
+// setter ran since it was built (the cache key covers every object's
+// generation stamp), creates the GPU context at the first solve.
+res_T prepare_scene(ssol_scene* scn) {
   ssol_device* dev = scn->dev;
-  if (!scn->sun) { log_error(dev, "ssol_solve: no sun attached.\n"); return RES_BAD_ARG; }
-  if (!scn->sun->spectrum || scn->sun->spectrum->wavelengths.empty()) {
-    log_error(dev, "ssol_solve: sun's spectrum undefined.\n");
-    return RES_BAD_ARG;
-  }
-  if (scn->sun->dni <= 0) { log_error(dev, "ssol_solve: sun's DNI undefined.\n"); return RES_BAD_ARG; }
-  if (tracker) {
-    log_error(dev, "ssol_solve: path tracking is not available on the GPU solver.\n");
-    return RES_BAD_ARG;
-  }
   std::vector<uint64_t> sig = scene_signature(scn);
   const bool cached = dev->cache && dev->cache->sig == sig;
   if (!cached) {
@@ -698,21 +702,18 @@ res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t
       return r;
     }
   }
-  Flat& f = dev->cache->flat;
-  Layout& L = dev->cache->layout;
-
   if (!dev->gpu) {
     const int e = ssolh_create(0, &dev->gpu);
     if (e) {
       log_error(dev, "ssol_solve: %s\n", ssolh_last_error());
       fprintf(stderr, "ssol_solve: GPU unavailable: %s\n", ssolh_last_error());
       dev->gpu = nullptr;
+      solve_cache_free(dev);
       return RES_UNKNOWN_ERR;
     }
   }
-  int e = 0;
   if (!cached) {
-    e = ssolh_scene_upload(dev->gpu, &f.scn);
+    const int e = ssolh_scene_upload(dev->gpu, &dev->cache->flat.scn);
     if (e) {
       fprintf(stderr, "ssol_solve: scene upload failed: %s\n", ssolh_last_error());
       solve_cache_free(dev);
@@ -720,6 +721,41 @@ res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t
     }
     dev->cache->sig.swap(sig);
   }
+  return RES_OK;
+}
+
+res_T check_sun(ssol_scene* scn, const char* who) {
+  ssol_device* dev = scn->dev;
+  if (!scn->sun) { log_error(dev, "%s: no sun attached.\n", who); return RES_BAD_ARG; }
+  if (!scn->sun->spectrum || scn->sun->spectrum->wavelengths.empty()) {
+    log_error(dev, "%s: sun's spectrum undefined.\n", who);
+    return RES_BAD_ARG;
+  }
+  if (scn->sun->dni <= 0) { log_error(dev, "%s: sun's DNI undefined.\n", who); return RES_BAD_ARG; }
+  return RES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t N,
+                 const size_t max_failed, const struct ssol_path_tracker* tracker,
+                 struct ssol_estimator** out) {
+  if (!scn || !rng || !N || !out) return RES_BAD_ARG;
+  *out = nullptr;
+  if (N > (size_t)INT64_MAX || max_failed > (size_t)INT64_MAX) return RES_BAD_ARG;
+  ssol_device* dev = scn->dev;
+  if (res_T rs = check_sun(scn, "ssol_solve")) return rs;
+  if (tracker) {
+    log_error(dev, "ssol_solve: path tracking is not available on the GPU solver.\n");
+    return RES_BAD_ARG;
+  }
+  res_T pr = prepare_scene(scn);
+  if (pr != RES_OK) return pr;
+  Flat& f = dev->cache->flat;
+  Layout& L = dev->cache->layout;
+  int e = 0;
   ssolh_solve_args a{};
   a.realisations = N;
   a.max_failed = max_failed;
@@ -728,6 +764,7 @@ res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t
   a.precision = dev->precision == SSOL_PRECISION_FP32 ? SSOLH_FP32 : SSOLH_FP64;
   a.path_slots = (uint32_t)std::min<size_t>(dev->path_slots, UINT32_MAX);
   a.pending_entries = 4;
+  a.graph_iterations = dev->graph_iterations;
   e = ssolh_solve(dev->gpu, &a);
   if (e) {
     fprintf(stderr, "ssol_solve: GPU solve failed: %s\n", ssolh_last_error());
@@ -969,6 +1006,119 @@ res_T ssol_mc_shape_get_mc_primitive(struct ssol_mc_shape* sh, const unsigned i,
     fields[f]->V /= area * area;
     fields[f]->SE /= area;
   }
+  return RES_OK;
+}
+
+res_T ssol_solve_sun_batch(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t nsuns,
+                           struct ssol_sun* const* suns, const size_t* realisations,
+                           struct ssol_mc_global* globals, size_t* counts) {
+  if (!scn || !rng || !nsuns || !suns || !realisations || !globals) return RES_BAD_ARG;
+  ssol_device* dev = scn->dev;
+  if (res_T rs = check_sun(scn, "ssol_solve_sun_batch")) return rs;
+  if (nsuns > (1u << 24)) return RES_BAD_ARG;
+  uint64_t total = 0;
+  for (size_t j = 0; j < nsuns; ++j) {
+    if (!suns[j] || realisations[j] > (size_t)INT64_MAX) return RES_BAD_ARG;
+    total += realisations[j];
+    if (total > (uint64_t)INT64_MAX) return RES_BAD_ARG;
+  }
+  if (!total) return RES_BAD_ARG;
+  res_T pr = prepare_scene(scn);
+  if (pr != RES_OK) return pr;
+  Flat& f = dev->cache->flat;
+  std::vector<ssolh_sun> dsuns(nsuns);
+  for (size_t j = 0; j < nsuns; ++j) {
+    if (flat_sun(f, suns[j], false, dsuns[j]) != RES_OK) {
+      log_error(dev, "ssol_solve_sun_batch: sun %zu is invalid or its spectrum is not the scene "
+                     "sun's.\n", j);
+      return RES_BAD_ARG;
+    }
+  }
+  std::vector<uint64_t> nr(realisations, realisations + nsuns);
+  ssolh_batch_args a{};
+  a.nsuns = (uint32_t)nsuns;
+  a.suns = dsuns.data();
+  a.realisations = nr.data();
+  a.max_failed = 0;
+  a.rng_key = rng->key;
+  a.rng_counter = rng->counter;
+  a.precision = dev->precision == SSOL_PRECISION_FP32 ? SSOLH_FP32 : SSOLH_FP64;
+  a.path_slots = (uint32_t)std::min<size_t>(dev->path_slots, UINT32_MAX);
+  a.pending_entries = 4;
+  a.graph_iterations = dev->graph_iterations;
+  std::vector<ssolh_batch_result> res(nsuns);
+  const int e = ssolh_solve_batch(dev->gpu, &a, res.data());
+  if (e) {
+    fprintf(stderr, "ssol_solve_sun_batch: GPU solve failed: %s\n", ssolh_last_error());
+    return e == SSOLH_ERR_ARG ? RES_BAD_ARG : RES_UNKNOWN_ERR;
+  }
+  for (size_t j = 0; j < nsuns; ++j) {
+    const double N = (double)res[j].realisation_count;
+    if (counts) counts[j] = (size_t)res[j].realisation_count;
+    if (N == 0) {  // a sun given no realisation: zero estimates
+      memset(&globals[j], 0, sizeof(globals[j]));
+      continue;
+    }
+    ssol_mc_result* fields[6] = {&globals[j].cos_factor, &globals[j].absorbed_by_receivers,
+                                 &globals[j].shadowed, &globals[j].missing,
+                                 &globals[j].extinguished_by_atmosphere,
+                                 &globals[j].other_absorbed};
+    for (int k = 0; k < 6; ++k) mc_result(res[j].global[k][0], res[j].global[k][1], N, fields[k]);
+  }
+  return RES_OK;
+}
+
+res_T ssol_device_autotune(struct ssol_device* dev, struct ssol_scene* scn, const size_t probe,
+                           struct ssol_autotune_result* out) {
+  if (!dev || !scn || scn->dev != dev || probe < 1024 || !out) return RES_BAD_ARG;
+  if (res_T rs = check_sun(scn, "ssol_device_autotune")) return rs;
+  res_T pr = prepare_scene(scn);
+  if (pr != RES_OK) return pr;
+  // candidates: path slots (the wavefront width) x iterations per graph
+  // launch (0: host-driven iterations); each timed on a probe solve after
+  // one warm-up, the fastest kept on the device
+  static const uint32_t slot_choices[] = {1u << 19, 1u << 20, 1u << 21, 1u << 22};
+  static const uint32_t graph_choices[] = {0, 8, 32};
+  double best = 1e300;
+  uint32_t best_slots = 0, best_graph = 0;
+  unsigned ntried = 0;
+  for (uint32_t sl : slot_choices) {
+    if (sl > 4 * probe && sl != slot_choices[0]) continue;
+    for (uint32_t gi : graph_choices) {
+      ssolh_solve_args a{};
+      a.realisations = probe;
+      a.rng_key = 0x6175746f74756e65ull;  // fixed key: the probe is only timed
+      a.precision = dev->precision == SSOL_PRECISION_FP32 ? SSOLH_FP32 : SSOLH_FP64;
+      a.path_slots = sl;
+      a.pending_entries = 4;
+      a.graph_iterations = gi;
+      double t = 0;
+      for (int rep = 0; rep < 2; ++rep) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const int e = ssolh_solve(dev->gpu, &a);
+        const auto t1 = std::chrono::steady_clock::now();
+        if (e) {
+          fprintf(stderr, "ssol_device_autotune: probe failed: %s\n", ssolh_last_error());
+          return RES_UNKNOWN_ERR;
+        }
+        t = std::chrono::duration<double>(t1 - t0).count();  // keep the second run
+      }
+      ++ntried;
+      if (t < best) { best = t; best_slots = sl; best_graph = gi; }
+    }
+  }
+  dev->path_slots = best_slots;
+  dev->graph_iterations = best_graph;
+  out->path_slots = best_slots;
+  out->graph_iterations = best_graph;
+  out->paths_per_second = (double)probe / best;
+  out->candidates = ntried;
+  return RES_OK;
+}
+
+res_T ssol_device_set_graph_iterations(struct ssol_device* dev, const unsigned n) {
+  if (!dev || n > 1024) return RES_BAD_ARG;
+  dev->graph_iterations = n;
   return RES_OK;
 }
 

@@ -149,6 +149,11 @@ class PunchedSurface(C.Structure):
                 ("nb_carvings", C.c_size_t)]
 
 
+class AutotuneResult(C.Structure):
+    _fields_ = [("path_slots", C.c_uint), ("graph_iterations", C.c_uint),
+                ("paths_per_second", C.c_double), ("candidates", C.c_uint)]
+
+
 class SolveStats(C.Structure):
     _fields_ = [("segments", C.c_uint64), ("paths", C.c_uint64), ("pending_spills", C.c_uint64),
                 ("kernel_ms", C.c_double), ("trace_ms", C.c_double), ("shade_ms", C.c_double),
@@ -165,6 +170,10 @@ def _declare(L):
         "ssol_device_ref_put": [VP],
         "ssol_device_set_precision": [VP, C.c_int],
         "ssol_device_set_wavefront_size": [VP, C.c_size_t],
+        "ssol_device_set_graph_iterations": [VP, C.c_uint],
+        "ssol_device_autotune": [VP, VP, C.c_size_t, C.POINTER(AutotuneResult)],
+        "ssol_solve_sun_batch": [VP, VP, C.c_size_t, C.POINTER(VP), C.POINTER(C.c_size_t),
+                                 C.POINTER(McGlobal), C.POINTER(C.c_size_t)],
         "ssol_scene_create": [VP, PVP],
         "ssol_scene_ref_put": [VP],
         "ssol_scene_attach_instance": [VP, VP],
@@ -303,7 +312,8 @@ def _normal_shader(kind):
 class Scene:
     """A scene built through the C API from a ssol_scene.Scene description."""
 
-    def __init__(self, spec: S.Scene, precision: str = "fp64", path_slots: int = 0):
+    def __init__(self, spec: S.Scene, precision: str = "fp64", path_slots: int = 0,
+                 graph_iterations: int = 0):
         self.spec = spec
         self._keep: List[object] = []
         self._handles: List[tuple] = []  # (release fn name, handle)
@@ -312,6 +322,8 @@ class Scene:
             call("ssol_device_set_precision", self.dev, 1)
         if path_slots:
             call("ssol_device_set_wavefront_size", self.dev, path_slots)
+        if graph_iterations:
+            call("ssol_device_set_graph_iterations", self.dev, graph_iterations)
         self.scn = new("ssol_scene_create", self.dev)
         self._spectra: Dict[int, C.c_void_p] = {}
         self._materials: Dict[int, C.c_void_p] = {}
@@ -326,20 +338,7 @@ class Scene:
             call("ssol_instance_sample", h, inst.sample)
             call("ssol_scene_attach_instance", self.scn, h)
             self.instances[id(inst)] = h
-        sun = spec.sun
-        ctor = {"directional": "ssol_sun_create_directional", "pillbox": "ssol_sun_create_pillbox",
-                "gaussian": "ssol_sun_create_gaussian", "buie": "ssol_sun_create_buie"}[sun.kind]
-        self.sun = new(ctor, self.dev)
-        self._handles.append(("ssol_sun_ref_put", self.sun))
-        call("ssol_sun_set_direction", self.sun, (C.c_double * 3)(*sun.direction))
-        call("ssol_sun_set_dni", self.sun, sun.dni)
-        call("ssol_sun_set_spectrum", self.sun, self._spectrum(sun.spectrum))
-        if sun.kind == "pillbox":
-            call("ssol_sun_pillbox_set_half_angle", self.sun, sun.param)
-        elif sun.kind == "gaussian":
-            call("ssol_sun_gaussian_set_std_dev", self.sun, sun.param)
-        elif sun.kind == "buie":
-            call("ssol_sun_set_buie_param", self.sun, sun.param)
+        self.sun = self._make_sun(spec.sun)
         call("ssol_scene_attach_sun", self.scn, self.sun)
         if spec.atmosphere is not None:
             self.atm = new("ssol_atmosphere_create", self.dev)
@@ -348,6 +347,45 @@ class Scene:
             call("ssol_atmosphere_set_extinction", self.atm, C.byref(d))
             lib().ssol_data_clear(C.byref(d))
             call("ssol_scene_attach_atmosphere", self.scn, self.atm)
+
+    def _make_sun(self, sun: S.Sun, direction=None):
+        ctor = {"directional": "ssol_sun_create_directional", "pillbox": "ssol_sun_create_pillbox",
+                "gaussian": "ssol_sun_create_gaussian", "buie": "ssol_sun_create_buie"}[sun.kind]
+        h = new(ctor, self.dev)
+        self._handles.append(("ssol_sun_ref_put", h))
+        d = sun.direction if direction is None else direction
+        call("ssol_sun_set_direction", h, (C.c_double * 3)(*d))
+        call("ssol_sun_set_dni", h, sun.dni)
+        call("ssol_sun_set_spectrum", h, self._spectrum(sun.spectrum))
+        if sun.kind == "pillbox":
+            call("ssol_sun_pillbox_set_half_angle", h, sun.param)
+        elif sun.kind == "gaussian":
+            call("ssol_sun_gaussian_set_std_dev", h, sun.param)
+        elif sun.kind == "buie":
+            call("ssol_sun_set_buie_param", h, sun.param)
+        return h
+
+    def solve_sun_batch(self, directions, counts, seed: int = 0, counter: int = 0):
+        """ssol_solve_sun_batch over suns like the scene's, pointing along
+        `directions`; returns a list of global-estimate dicts and counts."""
+        suns = [self._make_sun(self.spec.sun, d) for d in directions]
+        n = len(suns)
+        arr = (VP * n)(*[h.value if isinstance(h, C.c_void_p) else h for h in suns])
+        cnt = (C.c_size_t * n)(*[int(x) for x in counts])
+        out = (McGlobal * n)()
+        got = (C.c_size_t * n)()
+        r = Rng(seed, counter)
+        call("ssol_solve_sun_batch", self.scn, r.h, n, arr, cnt, out, got)
+        res = [{f: getattr(out[j], f).tuple() for f in GLOBAL_FIELDS} for j in range(n)]
+        for h in suns:
+            lib().ssol_sun_ref_put(h)
+            self._handles.remove(("ssol_sun_ref_put", h))
+        return res, list(got)
+
+    def autotune(self, probe: int) -> AutotuneResult:
+        out = AutotuneResult()
+        call("ssol_device_autotune", self.dev, self.scn, probe, C.byref(out))
+        return out
 
     # element builders
     def _spectrum(self, sp: S.Spectrum):

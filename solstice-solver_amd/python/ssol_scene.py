@@ -344,6 +344,20 @@ def config2(seed: int = 0, n_helio: int = 100, radius: float = 300.0,
     return Scene(insts + [rcv], sun, None, name=f"config2_n{n_helio}")
 
 
+def config4(seed: int = 0, n_suns: int = 512, n_az: int = 32, n_el: int = 16,
+            log_min: float = 5.0, log_max: float = 7.0):
+    """Batched sweep: the 100-paraboloid field (aimed for its default sun)
+    under n_az x n_el sun positions (azimuth 0-360 deg x elevation 5-85 deg)
+    with seeded log-uniform realisation counts in [10^log_min, 10^log_max].
+    Returns (scene, directions [n_suns][3], counts [n_suns])."""
+    rng = np.random.default_rng(seed)
+    az = np.linspace(0.0, 360.0, n_az, endpoint=False)
+    el = np.linspace(5.0, 85.0, n_el)
+    dirs = [sun_dir_from(a, e) for e in el for a in az][:n_suns]
+    counts = np.floor(10.0 ** rng.uniform(log_min, log_max, len(dirs))).astype(np.int64)
+    return config2(seed), np.array(dirs), counts
+
+
 def config3(seed: int = 0, n_helio: int = 10000, r_min: float = 100.0,
             r_max: float = 1500.0, sun_dir=None) -> Scene:
     """Beam-down field of 10,000 flat heliostats, each an 8x8-facet 4x4 m
