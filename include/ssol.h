@@ -657,6 +657,24 @@ SSOL_API res_T ssol_solve_sun_batch
    struct ssol_sun* const* suns, const size_t* realisations,
    struct ssol_mc_global* globals, size_t* counts);
 
+/* Adjoint of a solve: with the same scene, rng and realisations as a
+ * forward ssol_solve, returns per parameterised instance (heliostat p)
+ *   grad[p*3 + 0], grad[p*3 + 1]: derivatives with respect to rotations of
+ *     the instance about the world axes axes[p*6 + 0..2], axes[p*6 + 3..5];
+ *   grad[p*3 + 2]: derivative with respect to its glossy mirror's slope
+ *     error (Beckmann roughness = sqrt(2) slope error)
+ * of d_absorbed * absorbed_by_receivers + sum_t d_map[t] * F_t, F_t being
+ * the front-side per-primitive absorbed flux (W/m2) of map_shape on
+ * map_receiver (d_map may be NULL). Score-function estimator of the
+ * microfacet reflection plus the pathwise derivative of the start weight's
+ * cosine; blocking and shading changes of the moving instance are not
+ * differentiated. FP64 form only. */
+SSOL_API res_T ssol_solve_adjoint
+  (struct ssol_scene* scn, const struct ssp_rng* rng, const size_t realisations,
+   const size_t nparams, struct ssol_instance* const* heliostats, const double* axes,
+   const double d_absorbed, const struct ssol_instance* map_receiver,
+   const struct ssol_shape* map_shape, const double* d_map, double* grad);
+
 /* Autotuner: times probe solves of `probe` realisations over candidate
  * wavefront widths (path slots) and graph lengths, keeps the fastest as the
  * device's setting and reports it. */

@@ -326,6 +326,28 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* ctx, const struct ssolh_scene* s
 /* Run the realisations of one solve over the uploaded scene */
 SSOLH_API int ssolh_solve(ssolh_context* ctx, const struct ssolh_solve_args* args);
 
+/* Adjoint solve: the same realisations as the forward solve of `solve`
+ * (same RNG key and counter), with per-path score-function gradients of
+ *   d_absorbed * absorbed_by_receivers + sum_k gmap[k] * absorbed flux on
+ *   per-primitive tally key k (= (slot * 2 + side))
+ * with respect to, for each parameterised instance (heliostat), two aim
+ * angles (rotations about the world axes given in axes[p][0], axes[p][1])
+ * and the slope error of its glossy Beckmann mirror. FP64 form only. */
+struct ssolh_adjoint_args {
+  struct ssolh_solve_args solve;
+  uint32_t nparams;
+  uint32_t pad__;
+  const int32_t* inst_param;  /* [ninstances]: parameter slot or -1 */
+  const double* axes;         /* [nparams][2][3] */
+  const double* gmap;         /* [nprim_tallies][2] upstream weights, or NULL */
+  double d_absorbed;
+};
+
+/* grad: [nparams][3] = d/d angle 1, d/d angle 2, d/d slope error, per
+ * realisation (divided by the realisation count). */
+SSOLH_API int ssolh_solve_adjoint
+  (ssolh_context* ctx, const struct ssolh_adjoint_args* args, double* grad);
+
 /* Batched solve (see ssolh_batch_args); out: [nsuns] */
 SSOLH_API int ssolh_solve_batch
   (ssolh_context* ctx, const struct ssolh_batch_args* args,

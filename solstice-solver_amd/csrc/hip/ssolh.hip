@@ -217,6 +217,17 @@ __device__ __forceinline__ const ssolh_sun& job_sun(const DScene& S, const Solve
   return prm.job_suns ? prm.job_suns[job] : S.sun;
 }
 
+// Adjoint solve (score-function gradients, see k_shade): null grad <=> off.
+struct AdjParams {
+  double* grad;              // [nparams][3]: d/d aim angle 1, 2, d/d slope error
+  const int32_t* inst_param; // [ninstances]: parameter slot of an instance or -1
+  const double* axes;        // [nparams][2][3]: world rotation axes of the aim angles
+  const double* gmap;        // [nprim_tallies][2]: upstream weight per unit absorbed flux
+  double d_total;            // upstream weight of absorbed_by_receivers
+  double4* s;                // [slots]: scores of the path (aim 1, aim 2, slope), q
+  uint32_t* p;               // [slots]: parameter slot of the path or NONE
+};
+
 // Work list: slot of entry q (a null list is the identity)
 __device__ __forceinline__ uint32_t slot_of(const uint32_t* list, uint32_t q) {
   return list ? list[q] : q;
@@ -547,7 +558,8 @@ __device__ uint32_t start_medium(const DScene& S, const ssolh_material& m, uint3
 }
 
 __device__ void generate(const DScene& S, const PathState& P, const SolveParams& prm,
-                         uint32_t slot, uint64_t path, Emit& e, int& samp_slot) {
+                         uint32_t slot, uint64_t path, Emit& e, int& samp_slot,
+                         const AdjParams& A) {
   PathRng rng;
   rng.init(prm.key, prm.counter + path, 0);
   // area-weighted primitive through the alias table, uniform point
@@ -598,6 +610,19 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? is.mtl_front : is.mtl_back];
   const uint32_t med = start_medium(S, m, side);
 
+  if (A.grad) {
+    // w0 is proportional to |N.w|: d log w0 / d angle = ((a x N).w) / (N.w)
+    const int32_t pp = A.inst_param[st.instance];
+    double4 sc = make_double4(0, 0, 0, 0);
+    if (pp >= 0) {
+      const d3 a1 = ld3(A.axes + 6 * pp), a2 = ld3(A.axes + 6 * pp + 3);
+      sc.x = dot(cross(a1, N), dir) / surface_sun_cos;
+      sc.y = dot(cross(a2, N), dir) / surface_sun_cos;
+    }
+    A.s[slot] = sc;
+    A.p[slot] = pp >= 0 ? (uint32_t)pp : NONE;
+  }
+
   P.pos_wl[slot] = make_double4(pos.x, pos.y, pos.z, wl);
   P.dir_cos[slot] = make_double4(dir.x, dir.y, dir.z, cosf);
   P.prev_init[slot] = make_double4(w0, w0, w0, w0);
@@ -640,13 +665,13 @@ __device__ void flush_sampled_count(bool has, int samp_slot, const Tallies& T) {
 // runs realisations s + k * slots, k = 1, 2, ... while they are below N.
 __global__ void __launch_bounds__(BLOCK)
 k_generate(DScene S, PathState P, SolveParams prm, Tallies T, RayRec* __restrict__ rays,
-           uint32_t count) {
+           uint32_t count, AdjParams A) {
   const uint32_t slot = blockIdx.x * BLOCK + threadIdx.x;
   const bool live = slot < count;
   Emit e;
   e.has = false;
   int samp_slot = 0;
-  if (live) generate(S, P, prm, slot, slot, e, samp_slot);
+  if (live) generate(S, P, prm, slot, slot, e, samp_slot, A);
   flush_sampled_count(live, samp_slot, T);
   if (e.has) rays[slot] = e.r;
 }
@@ -697,7 +722,7 @@ __global__ void __launch_bounds__(BLOCK)
 k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __restrict__ list,
         const unsigned* __restrict__ count_ptr, RayRec* __restrict__ rays,
         const HitRec* __restrict__ hits, uint8_t* __restrict__ emitted,
-        uint32_t* __restrict__ block_emits, Counters* C) {
+        uint32_t* __restrict__ block_emits, Counters* C, AdjParams A) {
   const uint32_t count = *count_ptr;
   if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: block_count syncs
   const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
@@ -836,11 +861,33 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
         int lobe = LOBE_NONE;
         const d3 wi = -dir;
         if (fin.kind == FIN_NONE && dot(wi, N) > 0) {
-          d3 wo;
-          w = bsdf_sample(m, eta_out, eta_in, slab_eta, slab_abs, wi, N, rng, wo, lobe);
+          d3 wo, hm = N;
+          w = bsdf_sample(m, eta_out, eta_in, slab_eta, slab_abs, wi, N, rng, wo, lobe, &hm);
           const double c = dot(N, wo);
           if ((c > 0 && lobe == LOBE_TRANSMIT) || (c < 0 && lobe == LOBE_REFLECT)) w = 0;
           dir = wo;
+          if (A.grad && !(flags & PF_STAGE_BOUNCE) && w > 0 && m.type == SSOLH_MTL_MIRROR &&
+              m.roughness > 0 && m.distribution == 0) {
+            // first reflection of a path that starts on a parameterised
+            // heliostat: score of the Beckmann microfacet density
+            // p(h) = exp(-tan^2/m^2) / (pi m^2 cos^3) under a rotation of
+            // the macro normal N about axis a (dN = a x N) and under m:
+            //   dlog p/d angle = (2 sec^2/m^2 - 3) sec (h . (a x N))
+            //   dlog p/dm      = 2 tan^2/m^3 - 2/m,  m = sqrt(2) slope error
+            const uint32_t pp = A.p[slot];
+            if (pp != NONE) {
+              const double ct = fmin(dot(hm, N), 1.0);
+              const double sec = 1.0 / ct, t2 = sec * sec - 1.0;
+              const double mr = m.roughness;
+              const double k = (2.0 * sec * sec / (mr * mr) - 3.0) * sec;
+              const d3 a1 = ld3(A.axes + 6 * pp), a2 = ld3(A.axes + 6 * pp + 3);
+              double4 sc = A.s[slot];
+              sc.x += k * dot(hm, cross(a1, N));
+              sc.y += k * dot(hm, cross(a2, N));
+              sc.z += 1.4142135623730951 * (2.0 * t2 / (mr * mr * mr) - 2.0 / mr);
+              A.s[slot] = sc;
+            }
+          }
         }
         kabs = 1 - w;
         out0 = in0 * w;
@@ -867,7 +914,9 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
                                              (side == SSOL_FRONT_ ? 0u : 1u));
             if (!pend_add(P.qkey + slot, P.qval + slot, stride, prm.npend, pkey, w10))
               spill(T.prim, pkey, w10, &C->spills);
+            if (A.grad && A.p[slot] != NONE) A.s[slot].w += A.gmap[pkey] * w10[5];
           }
+          if (A.grad && A.p[slot] != NONE) A.s[slot].w += A.d_total * (in0 - out0);
         } else {
           pd.z += in0 * kabs;
         }
@@ -979,13 +1028,25 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
     }
   }
 
+  // --- adjoint: the finished path's weighted contribution times its scores
+  if (A.grad && done) {
+    const uint32_t pp = A.p[slot];
+    if (pp != NONE) {
+      const double4 sc = A.s[slot];
+      const double q = sc.w * fin.factor;
+      atomic_add(A.grad + 3 * (size_t)pp, q * sc.x);
+      atomic_add(A.grad + 3 * (size_t)pp + 1, q * sc.y);
+      atomic_add(A.grad + 3 * (size_t)pp + 2, q * sc.z);
+    }
+  }
+
   // --- refill finished slots with their next realisation -------------------
   const bool want = live && fin.kind != FIN_NONE && !C->aborted;
   const uint64_t id = want ? (((uint64_t)P.meta[slot].y << 32) | P.meta[slot].x) + prm.slots : 0;
   int samp_slot = 0;
   bool gen = false;
   if (want && id < prm.N) {
-    generate(S, P, prm, slot, id, e, samp_slot);
+    generate(S, P, prm, slot, id, e, samp_slot, A);
     gen = true;
   } else if (live && fin.kind != FIN_NONE) {
     P.meta[slot] = make_uint4(0, 0, 0, 0);
@@ -1160,6 +1221,8 @@ struct ssolh_context {
   uint32_t stack_depth = MAX_STACK;
   // solve buffers
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, counters, tallies, jobs;
+  DevBuf adj_state, adj_in, adj_grad;
+  uint32_t ninstances = 0;
   uint32_t njobs = 1;
   PathState P{};
   PathState32 P32{};
@@ -1289,7 +1352,8 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_scene(c);
   for (DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
-                    &c->block_emits, &c->block_offsets, &c->counters, &c->tallies, &c->jobs})
+                    &c->block_emits, &c->block_offsets, &c->counters, &c->tallies, &c->jobs,
+                    &c->adj_state, &c->adj_in, &c->adj_grad})
     if (b->p) (void)hipFree(b->p);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1361,6 +1425,7 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   c->nsamp = s->nsampled;
   c->nprim = s->nprim_tallies;
   c->stack_depth = s->stack_depth;
+  c->ninstances = s->ninstances;
   HIP_TRY(hipStreamSynchronize(c->stream));  // the host arrays may go after return
   c->has_scene = true;
   return 0;
@@ -1374,9 +1439,20 @@ namespace {
 // ids [0, N) map to jobs through job_first. graph_iters > 0 captures that
 // many iterations (rounded up to even) into a HIP graph replayed until the
 // work list is empty; 0 runs the iterations from the host one by one.
+// Host description of an adjoint solve (host pointers)
+struct AdjHost {
+  uint32_t nparams;
+  const int32_t* inst_param;  // [ninstances]
+  const double* axes;         // [nparams][2][3]
+  const double* gmap;         // [nprim_tallies][2] or null
+  double d_total;
+};
+
 int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, uint64_t counter,
               bool fp32, uint32_t path_slots, uint32_t npend, uint32_t njobs,
-              const uint64_t* job_first_h, const ssolh_sun* job_suns_h, uint32_t graph_iters) {
+              const uint64_t* job_first_h, const ssolh_sun* job_suns_h, uint32_t graph_iters,
+              const AdjHost* adj = nullptr) {
+  if (adj && fp32) { set_err("ssolh_solve_adjoint: the adjoint runs in the FP64 form"); return SSOLH_ERR_ARG; }
   if (npend > MAX_PEND) { set_err("ssolh_solve: at most %d pending entries", MAX_PEND); return SSOLH_ERR_LIMIT; }
   uint64_t slots64 = path_slots ? path_slots : (1u << 21);
   if (slots64 > N) slots64 = N;
@@ -1436,6 +1512,35 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   if ((r = dev_alloc(c->block_emits, max_blocks * sizeof(uint32_t)))) return r;
   if ((r = dev_alloc(c->block_offsets, max_blocks * sizeof(uint32_t)))) return r;
   if ((r = dev_alloc(c->counters, sizeof(Counters) + 4 * sizeof(unsigned)))) return r;
+  // adjoint buffers
+  AdjParams A{};
+  if (adj) {
+    const size_t ninst = c->ninstances, ngm = (size_t)c->nprim * 2;
+    const size_t in_bytes = ninst * sizeof(int32_t) + 64 + (size_t)adj->nparams * 6 * sizeof(double) +
+                            ngm * sizeof(double) + 64;
+    if ((r = dev_alloc(c->adj_state, n * (sizeof(double4) + sizeof(uint32_t)) + 256))) return r;
+    if ((r = dev_alloc(c->adj_in, in_bytes))) return r;
+    if ((r = dev_alloc(c->adj_grad, (size_t)adj->nparams * 3 * sizeof(double) + 8))) return r;
+    char* ai = (char*)c->adj_in.p;
+    int32_t* d_ip = (int32_t*)ai;
+    double* d_axes = (double*)(ai + (ninst * sizeof(int32_t) + 63) / 64 * 64);
+    double* d_gm = d_axes + (size_t)adj->nparams * 6;
+    HIP_TRY(hipMemcpyAsync(d_ip, adj->inst_param, ninst * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(d_axes, adj->axes, (size_t)adj->nparams * 6 * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+    if (adj->gmap && ngm)
+      HIP_TRY(hipMemcpyAsync(d_gm, adj->gmap, ngm * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    else if (ngm)
+      HIP_TRY(hipMemsetAsync(d_gm, 0, ngm * sizeof(double), c->stream));
+    HIP_TRY(hipMemsetAsync(c->adj_grad.p, 0, (size_t)adj->nparams * 3 * sizeof(double), c->stream));
+    A.grad = (double*)c->adj_grad.p;
+    A.inst_param = d_ip;
+    A.axes = d_axes;
+    A.gmap = d_gm;
+    A.d_total = adj->d_total;
+    A.s = (double4*)c->adj_state.p;
+    A.p = (uint32_t*)((char*)c->adj_state.p + (n * sizeof(double4) + 255) / 256 * 256);
+  }
   // jobs
   if (job_suns_h) {
     if ((r = dev_alloc(c->jobs, (njobs + 1) * sizeof(uint64_t) + njobs * sizeof(ssolh_sun)))) return r;
@@ -1532,7 +1637,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     else
       k_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
                                                (RayRec*)c->rays.p, (const HitRec*)c->hits.p,
-                                               emitted, block_emits, ctr);
+                                               emitted, block_emits, ctr, A);
     HIP_TRY(hipGetLastError());
     if (!graph_iters) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
     k_scan<<<1, SCAN_THREADS, 0, c->stream>>>(block_emits, count, block_offsets, cnt + (i ^ 1),
@@ -1554,7 +1659,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
         c->scn, c->shapeq, c->P32, prm, c->T, (RayRec32*)c->rays.p, count);
   else
     k_generate<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
-        c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count);
+        c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A);
   HIP_TRY(hipGetLastError());
   uint32_t iter = 0;
   if (!graph_iters) {
@@ -1692,6 +1797,39 @@ SSOLH_API int ssolh_solve_batch(ssolh_context* c, const struct ssolh_batch_args*
     memcpy(out[j].global, &g[(size_t)j * 16], 12 * sizeof(double));
     out[j].realisation_count = (uint64_t)g[(size_t)j * 16 + 12];
   }
+  return 0;
+}
+
+SSOLH_API int ssolh_solve_adjoint(ssolh_context* c, const struct ssolh_adjoint_args* a,
+                                  double* grad) {
+  if (!c || !a || !grad || !a->solve.realisations || !a->nparams || !a->inst_param || !a->axes) {
+    set_err("ssolh_solve_adjoint: invalid arguments");
+    return SSOLH_ERR_ARG;
+  }
+  if (!c->has_scene) { set_err("ssolh_solve_adjoint: no scene uploaded"); return SSOLH_ERR_ARG; }
+  if (a->solve.precision != SSOLH_FP64) {
+    set_err("ssolh_solve_adjoint: the adjoint runs in the FP64 form");
+    return SSOLH_ERR_ARG;
+  }
+  for (uint32_t i = 0; i < c->ninstances; ++i) {
+    if (a->inst_param[i] >= (int32_t)a->nparams) {
+      set_err("ssolh_solve_adjoint: instance %u has parameter slot %d >= %u", i, a->inst_param[i],
+              a->nparams);
+      return SSOLH_ERR_ARG;
+    }
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  AdjHost h{a->nparams, a->inst_param, a->axes, a->gmap, a->d_absorbed};
+  const uint32_t npend = a->solve.pending_entries ? a->solve.pending_entries : 4;
+  int r = run_solve(c, a->solve.realisations, a->solve.max_failed, a->solve.rng_key,
+                    a->solve.rng_counter, false, a->solve.path_slots, npend, 1, nullptr, nullptr,
+                    a->solve.graph_iterations, &h);
+  if (r) return r;
+  HIP_TRY(hipMemcpy(grad, c->adj_grad.p, (size_t)a->nparams * 3 * sizeof(double),
+                    hipMemcpyDeviceToHost));
+  double n = 0;
+  HIP_TRY(hipMemcpy(&n, c->T.global + 12, sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < (size_t)a->nparams * 3; ++i) grad[i] = n > 0 ? grad[i] / n : 0.0;
   return 0;
 }
 

@@ -345,10 +345,11 @@ SD d3 sample_microfacet(int distrib, double m, double u1, double u2, d3 n) {
 SD d3 reflect(d3 wi, d3 n, double cos_i) { return n * (2 * cos_i) - wi; }
 
 // wi points away from the surface on n's side. Returns the MC weight.
+// h_out (optional): the microfacet normal of a glossy reflection.
 template <class Rng>
 SD double bsdf_sample(const ssolh_material& m, double eta_out, double eta_in,
                       double slab_eta, double slab_abs, d3 wi, d3 n,
-                      Rng& rng, d3& wo, int& lobe) {
+                      Rng& rng, d3& wo, int& lobe, d3* h_out = nullptr) {
   const double cos_i = dot(wi, n);
   lobe = LOBE_REFLECT;
   switch (m.type) {
@@ -363,7 +364,10 @@ SD double bsdf_sample(const ssolh_material& m, double eta_out, double eta_in,
         const double c = dot(wi, h);
         if (c <= 0) continue;
         wo = reflect(wi, h, c);
-        if (dot(wo, n) > 0) return m.reflectivity;
+        if (dot(wo, n) > 0) {
+          if (h_out) *h_out = h;
+          return m.reflectivity;
+        }
       }
       wo = n;
       return 0;

@@ -1068,6 +1068,77 @@ res_T ssol_solve_sun_batch(struct ssol_scene* scn, const struct ssp_rng* rng, co
   return RES_OK;
 }
 
+res_T ssol_solve_adjoint(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t N,
+                         const size_t nparams, struct ssol_instance* const* heliostats,
+                         const double* axes, const double d_absorbed,
+                         const struct ssol_instance* map_receiver,
+                         const struct ssol_shape* map_shape, const double* d_map,
+                         double* grad) {
+  if (!scn || !rng || !N || !nparams || !heliostats || !axes || !grad) return RES_BAD_ARG;
+  if (N > (size_t)INT64_MAX || nparams > (1u << 24)) return RES_BAD_ARG;
+  if (d_map && (!map_receiver || !map_shape)) return RES_BAD_ARG;
+  ssol_device* dev = scn->dev;
+  if (res_T rs = check_sun(scn, "ssol_solve_adjoint")) return rs;
+  if (dev->precision != SSOL_PRECISION_FP64) {
+    log_error(dev, "ssol_solve_adjoint: the adjoint runs in the FP64 form.\n");
+    return RES_BAD_ARG;
+  }
+  res_T pr = prepare_scene(scn);
+  if (pr != RES_OK) return pr;
+  Layout& L = dev->cache->layout;
+  // parameter slot of every scene instance (flat instances follow the
+  // scene's instance order)
+  std::vector<int32_t> inst_param(scn->instances.size(), -1);
+  for (size_t p = 0; p < nparams; ++p) {
+    const ssol_instance* h = heliostats[p];
+    size_t i = 0;
+    while (i < scn->instances.size() && scn->instances[i] != h) ++i;
+    if (i == scn->instances.size() || inst_param[i] >= 0) return RES_BAD_ARG;
+    inst_param[i] = (int32_t)p;
+  }
+  std::vector<double> gmap;
+  if (d_map) {
+    auto it = L.prim_first.find({map_receiver->id, map_shape->uid});
+    if (it == L.prim_first.end()) {
+      log_error(dev, "ssol_solve_adjoint: the map shape has no per-primitive tallies.\n");
+      return RES_BAD_ARG;
+    }
+    gmap.assign(L.nprim * 2, 0.0);
+    const size_t ntris = map_shape->ids.size() / 3;
+    for (size_t t = 0; t < ntris; ++t) {
+      // the map is in W/m2 (the getter divides by the triangle's area)
+      const float* v0 = &map_shape->pos[3 * map_shape->ids[3 * t]];
+      const float* v1 = &map_shape->pos[3 * map_shape->ids[3 * t + 1]];
+      const float* v2 = &map_shape->pos[3 * map_shape->ids[3 * t + 2]];
+      const double e0[3] = {(double)v1[0] - v0[0], (double)v1[1] - v0[1], (double)v1[2] - v0[2]};
+      const double e1[3] = {(double)v2[0] - v0[0], (double)v2[1] - v0[1], (double)v2[2] - v0[2]};
+      const double n[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2],
+                           e0[0] * e1[1] - e0[1] * e1[0]};
+      const double area = 0.5 * std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      gmap[(it->second + t) * 2] = area > 0 ? d_map[t] / area : 0.0;  // front side
+    }
+  }
+  ssolh_adjoint_args a{};
+  a.solve.realisations = N;
+  a.solve.rng_key = rng->key;
+  a.solve.rng_counter = rng->counter;
+  a.solve.precision = SSOLH_FP64;
+  a.solve.path_slots = (uint32_t)std::min<size_t>(dev->path_slots, UINT32_MAX);
+  a.solve.pending_entries = 4;
+  a.solve.graph_iterations = dev->graph_iterations;
+  a.nparams = (uint32_t)nparams;
+  a.inst_param = inst_param.data();
+  a.axes = axes;
+  a.gmap = gmap.empty() ? nullptr : gmap.data();
+  a.d_absorbed = d_absorbed;
+  const int e = ssolh_solve_adjoint(dev->gpu, &a, grad);
+  if (e) {
+    fprintf(stderr, "ssol_solve_adjoint: GPU solve failed: %s\n", ssolh_last_error());
+    return e == SSOLH_ERR_ARG ? RES_BAD_ARG : RES_UNKNOWN_ERR;
+  }
+  return RES_OK;
+}
+
 res_T ssol_device_autotune(struct ssol_device* dev, struct ssol_scene* scn, const size_t probe,
                            struct ssol_autotune_result* out) {
   if (!dev || !scn || scn->dev != dev || probe < 1024 || !out) return RES_BAD_ARG;

@@ -172,6 +172,9 @@ def _declare(L):
         "ssol_device_set_wavefront_size": [VP, C.c_size_t],
         "ssol_device_set_graph_iterations": [VP, C.c_uint],
         "ssol_device_autotune": [VP, VP, C.c_size_t, C.POINTER(AutotuneResult)],
+        "ssol_solve_adjoint": [VP, VP, C.c_size_t, C.c_size_t, C.POINTER(VP),
+                               C.POINTER(C.c_double), C.c_double, VP, VP,
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "ssol_solve_sun_batch": [VP, VP, C.c_size_t, C.POINTER(VP), C.POINTER(C.c_size_t),
                                  C.POINTER(McGlobal), C.POINTER(C.c_size_t)],
         "ssol_scene_create": [VP, PVP],
@@ -381,6 +384,29 @@ class Scene:
             lib().ssol_sun_ref_put(h)
             self._handles.remove(("ssol_sun_ref_put", h))
         return res, list(got)
+
+    def solve_adjoint(self, n: int, heliostats, axes, d_absorbed: float = 0.0,
+                      map_receiver=None, map_shape=None, d_map=None, seed: int = 0,
+                      counter: int = 0) -> np.ndarray:
+        """ssol_solve_adjoint: gradient [len(heliostats)][3] (two aim angles
+        about `axes` [p][2][3], slope error) of d_absorbed * absorbed +
+        d_map . per-primitive absorbed flux map (W/m2) of map_shape on
+        map_receiver, from the realisations of solve(n, seed, counter)."""
+        hs = (VP * len(heliostats))(*[self.instances[id(h)].value for h in heliostats])
+        ax = np.ascontiguousarray(axes, dtype=np.float64).reshape(-1)
+        assert ax.size == 6 * len(heliostats)
+        grad = np.zeros((len(heliostats), 3))
+        dm = None
+        if d_map is not None:
+            dm = np.ascontiguousarray(d_map, dtype=np.float64)
+        r = Rng(seed, counter)
+        call("ssol_solve_adjoint", self.scn, r.h, n, len(heliostats), hs,
+             ax.ctypes.data_as(C.POINTER(C.c_double)), float(d_absorbed),
+             self.instances[id(map_receiver)] if map_receiver is not None else None,
+             self.shapes[id(map_shape)] if map_shape is not None else None,
+             dm.ctypes.data_as(C.POINTER(C.c_double)) if dm is not None else None,
+             grad.ctypes.data_as(C.POINTER(C.c_double)))
+        return grad
 
     def autotune(self, probe: int) -> AutotuneResult:
         out = AutotuneResult()

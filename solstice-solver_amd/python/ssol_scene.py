@@ -313,7 +313,8 @@ def sun_dir_from(azimuth_deg: float, elevation_deg: float) -> np.ndarray:
 def config2(seed: int = 0, n_helio: int = 100, radius: float = 300.0,
             sun_dir=None, slope_error: float = 2e-3, sigma_sun: float = 2.51e-3,
             aim_perturb: Optional[np.ndarray] = None,
-            slope_errors: Optional[Sequence[float]] = None) -> Scene:
+            slope_errors: Optional[Sequence[float]] = None,
+            rcv_size: float = 10.0, rcv_cells: int = 32) -> Scene:
     """Radial-staggered field of paraboloid heliostats (seeded positions
     within `radius`), focal length = slant range, carved to 6x6 m, Beckmann
     glossy mirror of the given slope error (roughness = sqrt(2) * slope error,
@@ -325,7 +326,7 @@ def config2(seed: int = 0, n_helio: int = 100, radius: float = 300.0,
     if sun_dir is None:
         sun_dir = sun_dir_from(180.0, 60.0)
     tower = np.array([0.0, 0.0, 100.0])
-    rcv_mesh = grid_mesh(10.0, 10.0, 32, 32)
+    rcv_mesh = grid_mesh(rcv_size, rcv_size, rcv_cells, rcv_cells)
     rcv_obj = Object([(rcv_mesh, matte(0.0), matte(0.0))])
     rcv = Instance(rcv_obj, transform(rot_x(math.pi), tower), receiver_mask=FRONT,
                    per_primitive=1, sample=0)
@@ -342,6 +343,39 @@ def config2(seed: int = 0, n_helio: int = 100, radius: float = 300.0,
         insts.append(Instance(obj, transform(R, pos), sample=1))
     sun = Sun("gaussian", sun_dir, 1000.0, mono_spectrum(), param=sigma_sun)
     return Scene(insts + [rcv], sun, None, name=f"config2_n{n_helio}")
+
+
+def aim_axes(seed: int = 0, n_helio: int = 100, radius: float = 300.0, sun_dir=None,
+             aim_perturb: Optional[np.ndarray] = None) -> np.ndarray:
+    """World rotation axes [n][2][3] of the two aim angles of config2's
+    heliostats, whose rotation is R rot_x(a0) rot_y(a1): d/da0 rotates about
+    R e_x and d/da1 about R rot_x(a0) e_y."""
+    rng = np.random.default_rng(seed)
+    pts = _radial_staggered(n_helio, 30.0, radius, rng)
+    if sun_dir is None:
+        sun_dir = sun_dir_from(180.0, 60.0)
+    tower = np.array([0.0, 0.0, 100.0])
+    out = np.zeros((n_helio, 2, 3))
+    for i, (x, y) in enumerate(pts):
+        R, _ = _aim(np.array([x, y, 0.0]), tower, sun_dir)
+        a0 = 0.0 if aim_perturb is None else float(aim_perturb[i, 0])
+        out[i, 0] = R @ np.array([1.0, 0.0, 0.0])
+        out[i, 1] = R @ rot_x(a0) @ np.array([0.0, 1.0, 0.0])
+    return out
+
+
+def config5(seed: int = 0, n_helio: int = 1000, radius: float = 900.0,
+            aim_range: float = 5e-3, slope_range=(1e-3, 3e-3), **kw):
+    """Differentiable-aiming field: config2's paraboloid heliostats with
+    seeded uniform aim perturbations in [-aim_range, aim_range] (two angles
+    each) and slope errors uniform in slope_range. Returns (scene, aim
+    [n][2], slope errors [n], aim axes [n][2][3])."""
+    rng = np.random.default_rng(seed + 5)
+    aim = rng.uniform(-aim_range, aim_range, (n_helio, 2))
+    slope = rng.uniform(slope_range[0], slope_range[1], n_helio)
+    spec = config2(seed, n_helio=n_helio, radius=radius, aim_perturb=aim, slope_errors=slope, **kw)
+    axes = aim_axes(seed, n_helio, radius, kw.get("sun_dir"), aim)
+    return spec, aim, slope, axes
 
 
 def config4(seed: int = 0, n_suns: int = 512, n_az: int = 32, n_el: int = 16,

@@ -1,0 +1,120 @@
+"""Hand-written adjoint of the flux integrator (ssol_solve_adjoint) against
+common-random-number (CRN) central differences, h = 1e-4, of the GPU solver
+and of the oracle; gradients of the absorbed flux and of the receiver flux
+map's variance with respect to heliostat aim angles and slope errors; and
+a 50-step gradient descent on flux uniformity through torch autograd.
+
+Statistics: adjoint and FD are each averaged over independent seeds; the
+check is |adjoint - FD| <= 2% |FD| + 3 combined standard errors, on the
+parameters with the largest gradients, and each compared FD mean must be
+resolved to better than 15% (so the check is not vacuous)."""
+import numpy as np
+import pytest
+import torch
+
+import adjoint_harness as H
+import ssol_scene as S
+
+pytestmark = pytest.mark.gpu
+
+H_FD = 1e-4
+SEEDS = (11, 12, 13, 14)
+
+
+def _params(fld, seed=0):
+    rng = np.random.default_rng(seed + 5)
+    aim = rng.uniform(-5e-3, 5e-3, (fld.n_helio, 2))
+    slope = rng.uniform(1e-3, 3e-3, fld.n_helio)
+    return aim, slope
+
+
+def _mean_se(xs):
+    xs = np.asarray(xs, dtype=np.float64)
+    return xs.mean(axis=0), xs.std(axis=0, ddof=1) / np.sqrt(len(xs))
+
+
+def _check(name, adj, fd):
+    (ma, sa), (mf, sf) = _mean_se(adj), _mean_se(fd)
+    tol = 0.02 * abs(mf) + 3 * np.hypot(sa, sf)
+    print(f"{name}: adjoint {ma:.6g} +- {sa:.3g}, FD {mf:.6g} +- {sf:.3g}, tol {tol:.3g}")
+    assert sf <= 0.15 * abs(mf), (name, "FD not resolved", mf, sf)
+    assert abs(ma - mf) <= tol, (name, ma, mf, tol)
+
+
+@pytest.fixture(scope="module")
+def small_field(gpu):
+    fld = H.Field(n_helio=30, rcv_size=3.0, rcv_cells=16, n=4_000_000)
+    aim, slope = _params(fld)
+    return fld, aim, slope
+
+
+def test_adjoint_absorbed_vs_crn_fd(small_field):
+    fld, aim, slope = small_field
+    pilot = H.adjoint(fld, aim, slope, 1.0, None, seed=3)
+    picks = [np.unravel_index(j, pilot[:, :2].shape)
+             for j in np.argsort(-np.abs(pilot[:, :2]).ravel())[:3]]
+    picks.append((int(np.argmax(np.abs(pilot[:, 2]))), 2))
+    grads = [H.adjoint(fld, aim, slope, 1.0, None, seed=s) for s in SEEDS]
+    for (i, k) in picks:
+        fd = [H.crn_fd(fld, aim, slope, i, k, H_FD, lambda a, m: a, seed=s) for s in SEEDS]
+        _check(f"d absorbed / d p[{i},{k}]", [g[i, k] for g in grads], fd)
+
+
+def test_adjoint_map_variance_vs_crn_fd(small_field):
+    fld, aim, slope = small_field
+    objective = lambda a, m: H.map_variance(m)  # noqa: E731
+    adj, picks = [], None
+    for s in SEEDS:
+        _, fmap = H.forward(fld, aim, slope, seed=s)
+        d_map = 2.0 * (fmap - fmap.mean()) / fmap.size
+        g = H.adjoint(fld, aim, slope, 0.0, d_map, seed=s)
+        adj.append(g)
+        if picks is None:
+            picks = [np.unravel_index(j, g[:, :2].shape)
+                     for j in np.argsort(-np.abs(g[:, :2]).ravel())[:3]]
+    for (i, k) in picks:
+        fd = [H.crn_fd(fld, aim, slope, i, k, H_FD, objective, seed=s) for s in SEEDS]
+        _check(f"d var(map) / d p[{i},{k}]", [g[i, k] for g in adj], fd)
+
+
+def test_adjoint_vs_oracle_fd(small_field):
+    """CRN differences of the oracle (the CPU restatement of the reference
+    solver) on the largest aim gradient of the absorbed flux."""
+    fld, aim, slope = small_field
+    grads = [H.adjoint(fld, aim, slope, 1.0, None, seed=s) for s in SEEDS]
+    mean = np.mean(grads, axis=0)
+    i, k = np.unravel_index(int(np.argmax(np.abs(mean[:, :2]))), mean[:, :2].shape)
+    fd = [H.crn_fd(fld, aim, slope, i, k, H_FD, lambda a, m: a, seed=s, n=2_000_000,
+                   solver="oracle") for s in (21, 22, 23)]
+    _check(f"oracle d absorbed / d p[{i},{k}]", [g[i, k] for g in grads], fd)
+
+
+def test_autograd_function_and_descent(gpu):
+    """torch.autograd drives the adjoint: the gradient of a scalar loss of
+    (absorbed, map) equals the adjoint with the loss's upstream weights, and
+    50 steps of Adam on the map variance flatten the flux map."""
+    fld = H.Field(n_helio=12, rcv_size=4.0, rcv_cells=16, n=1_000_000)
+    aim0, slope0 = _params(fld, seed=2)
+    aim = torch.tensor(aim0, requires_grad=True)
+    slope = torch.tensor(slope0, requires_grad=False)
+    absorbed, fmap = H.FluxFunction.apply(aim, slope, fld)
+    loss = H.map_variance(fmap) - 1e-3 * absorbed
+    loss.backward()
+    d_map = (2.0 * (fmap - fmap.mean()) / fmap.numel()).detach().numpy()
+    g = H.adjoint(fld, aim0, slope0, -1e-3, d_map)
+    assert np.allclose(aim.grad.numpy(), g[:, :2], rtol=1e-12, atol=0)
+
+    opt = torch.optim.Adam([aim], lr=1e-3)
+    v0 = None
+    for step in range(50):
+        opt.zero_grad()
+        _, fmap = H.FluxFunction.apply(aim, slope, fld)
+        v = H.map_variance(fmap)
+        if v0 is None:
+            v0 = float(v)
+        v.backward()
+        opt.step()
+    _, fmap = H.FluxFunction.apply(aim.detach(), slope, fld)
+    v1 = float(H.map_variance(fmap))
+    print(f"map variance {v0:.6g} -> {v1:.6g} after 50 steps")
+    assert v1 < 0.8 * v0
