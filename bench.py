@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "solstice-solver_amd", "python")]
 
@@ -98,6 +100,98 @@ def load_traffic(kernel, workload):
         return json.load(f)
 
 
+def other_forms(spec, B, ttt_batch, args):
+    """The FP32 form and the graph-captured FP64 form on the same field:
+    Mpaths/s, time to 0.1% rel. SE on absorbed, and the FP32 estimate's
+    difference from an FP64 run of the same size (in combined SE)."""
+    import ssol_ctypes as L
+    out = {}
+    ref = None
+    for name, prec, graph in (("fp64_graph", "fp64", 16), ("fp32", "fp32", 0),
+                              ("fp32_graph", "fp32", 16)):
+        lib = L.Scene(spec, precision=prec, graph_iterations=graph)
+        lib.solve(B, seed=200)  # warm-up (flatten, upload, allocation)
+        t0 = time.perf_counter()
+        est = lib.solve(B, seed=201)
+        dt = time.perf_counter() - t0
+        g = est.global_()["absorbed_by_receivers"]
+        entry = {"value": round(B / dt / 1e6, 3), "unit": "Mpaths/s"}
+        if prec == "fp64":
+            ref = g
+        elif ref is not None:
+            entry["absorbed_rel_diff_vs_fp64"] = (g[0] - ref[0]) / ref[0]
+            entry["absorbed_diff_in_combined_se"] = (g[0] - ref[0]) / math.hypot(g[2], ref[2])
+        if not args.no_ttt:
+            t, acc = time_to_rel_se(lib, batch=ttt_batch)
+            entry["time_to_0.1pct_rel_se_s"] = round(t, 4)
+        out[name] = entry
+        print(f"[bench] {name}: {entry}", file=sys.stderr, flush=True)
+        lib.close()
+    return out
+
+
+def sweep(n_suns=512, log_min=5.0, log_max=7.0, seq_suns=64):
+    """The 100-paraboloid field under n_suns sun positions with seeded
+    log-uniform realisation counts: one batched solve (graph-captured) and,
+    for comparison, the first seq_suns suns as separate ssol_solve calls
+    through the public API (a sun change re-flattens the scene, as any
+    caller's per-sun solve would)."""
+    import ssol_ctypes as L
+    import ssol_scene as S
+    spec, dirs, counts = S.config4(n_suns=n_suns, log_min=log_min, log_max=log_max)
+    lib = L.Scene(spec, graph_iterations=16)
+    lib.solve_sun_batch(dirs[:4], [10000] * 4, seed=1)  # warm-up
+    t0 = time.perf_counter()
+    res, got = lib.solve_sun_batch(dirs, counts, seed=2)
+    t_batch = time.perf_counter() - t0
+    lib.close()
+    n_total = int(sum(int(c) for c in counts))
+    # sequential: one scene per sun position (the reference's workflow)
+    t0 = time.perf_counter()
+    n_seq = 0
+    for j in range(seq_suns):
+        one = S.Scene(spec.instances, S.Sun(spec.sun.kind, dirs[j], spec.sun.dni,
+                                            spec.sun.spectrum, spec.sun.param),
+                      spec.atmosphere, name="one")
+        sl = L.Scene(one)
+        sl.solve(int(counts[j]), seed=3)
+        sl.close()
+        n_seq += int(counts[j])
+    t_seq = time.perf_counter() - t0
+    out = {"suns": n_suns, "paths": n_total, "batched_s": round(t_batch, 4),
+           "batched_Mpaths_s": round(n_total / t_batch / 1e6, 2),
+           "sequential_sample_suns": seq_suns, "sequential_sample_paths": n_seq,
+           "sequential_sample_s": round(t_seq, 4),
+           "sequential_Mpaths_s": round(n_seq / t_seq / 1e6, 2),
+           "speedup_vs_sequential": round((n_total / t_batch) / (n_seq / t_seq), 2)}
+    print(f"[bench] sweep: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def adjoint_timing(n=20_000_000):
+    """The 1,000-heliostat differentiable-aiming field: forward solve and
+    adjoint (aim angles and slope errors of every heliostat) of the same
+    realisations."""
+    import ssol_ctypes as L
+    import ssol_scene as S
+    spec, aim, slope, axes = S.config5()
+    lib = L.Scene(spec)
+    lib.solve(1_000_000, seed=1)
+    t0 = time.perf_counter()
+    est = lib.solve(n, seed=2)
+    t_fwd = time.perf_counter() - t0
+    lib.solve_adjoint(1_000_000, spec.instances[:-1], axes, d_absorbed=1.0, seed=1)
+    t0 = time.perf_counter()
+    g = lib.solve_adjoint(n, spec.instances[:-1], axes, d_absorbed=1.0, seed=2)
+    t_adj = time.perf_counter() - t0
+    lib.close()
+    out = {"heliostats": len(spec.instances) - 1, "paths": n, "forward_s": round(t_fwd, 4),
+           "adjoint_s": round(t_adj, 4), "adjoint_Mpaths_s": round(n / t_adj / 1e6, 2),
+           "grad_norm": float(np.linalg.norm(g))}
+    print(f"[bench] adjoint: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(spec, seconds: float, threads: int):
     """The oracle (FP64 OpenMP restatement of the reference solver) on the
     host cores, on a bounded sample of the same workload."""
@@ -132,6 +226,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth yardstick")
     ap.add_argument("--no-ttt", action="store_true", help="skip the time-to-0.1%% run")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the FP32 / graph-captured forms, the sweep and the adjoint")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -226,6 +322,11 @@ def main(argv=None):
         out["paths_to_0.1pct_rel_se"] = acc[2]
         print(f"[bench] 0.1% rel. SE on absorbed after {acc[2]} paths in {dt:.3f} s",
               file=sys.stderr, flush=True)
+
+    if not args.no_forms and args.config == "c3":
+        out["forms"] = other_forms(spec, B, min(B, 4_000_000), args)
+        out["sweep_512_suns"] = sweep()
+        out["adjoint_1000_heliostats"] = adjoint_timing()
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
