@@ -379,6 +379,11 @@ SSOLH_API int ssolh_test_threefry
   (ssolh_context* ctx, uint32_t n, const uint64_t* ctr, const uint64_t* key,
    uint64_t* out);
 
+/* Wave-wide sum used by the tallies: out[w*64 + i] = sum of in[w*64 ..
+ * w*64+63] for every lane i, nwaves waves of 64 lanes */
+SSOLH_API int ssolh_test_wave_sum
+  (ssolh_context* ctx, uint32_t nwaves, const double* in, double* out);
+
 /* Bandwidth of a plain device copy of `bytes` bytes repeated `reps` times
  * (read + write bytes over the HIP-event time), in GB/s. Calibrates the HBM
  * byte counters of the profiler; no counterpart in the reference. */

@@ -172,18 +172,19 @@ struct Counters {
 // Tallies that every wave updates (global fields, counts) or that few keys
 // share (receivers, sampled instances) are striped: block b adds into
 // stripe b & mask, so concurrent atomics spread over many cache lines; the
-// stripes are folded into stripe 0 after the solve. Keys that are many and
-// scattered (sampled x receiver, per primitive) have one copy.
+// stripes are folded into stripe 0 after the solve. The stripe count of a
+// tally shrinks with its size (2^21 doubles at most over all its stripes).
 struct Tallies {
   double* global;   // [mask_g+1][njobs][16]: 6 fields x {w, w2}, then the
                     // realisation count (exact below 2^53); one 128-byte line
   double* rcv;      // [mask_r+1][nrcv][2][10][2]
   double* samp;     // [mask_s+1][nsamp][2][2]
   unsigned long long* samp_count;    // [mask_s+1][nsamp]
-  double* sxr;      // [nsamp][nrcv][2][10][2]
-  double* prim;     // [nprim][2][10][2]
-  uint32_t mask_g, mask_r, mask_s;
+  double* sxr;      // [mask_x+1][nsamp][nrcv][2][10][2]
+  double* prim;     // [mask_p+1][nprim][2][10][2]
+  uint32_t mask_g, mask_r, mask_s, mask_x, mask_p;
   uint32_t nrcv, nsamp, njobs;
+  uint64_t nprim;
 };
 
 struct SolveParams {
@@ -236,10 +237,32 @@ __device__ __forceinline__ uint32_t slot_of(const uint32_t* list, uint32_t q) {
 // -------------------------------------------------------------------------
 // Wave and block helpers (64-wide waves)
 // -------------------------------------------------------------------------
+// Value of lane (i ^ pattern) within DPP rows, both halves of a double
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Sum over the 64 lanes of a wave; every lane must be active. DPP moves
+// sum within quads (quad_perm [1,0,3,2], [2,3,0,1]), half rows
+// (row_half_mirror) and rows of 16 (row_mirror); the four row sums are
+// then read from lanes 0, 16, 32, 48.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  v += dpp_move<0x140>(v);
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
@@ -259,12 +282,19 @@ __device__ void keyed_add(bool has, uint64_t key, const double* v, double* base)
   const int leader = __ffsll((unsigned long long)mask) - 1;
   const uint64_t lkey = __shfl(key, leader, 64);
   const bool match = has && key == lkey;
+  if (__popcll(__ballot(match)) > 1) {
+    // several lanes share the key: one reduction, one set of atomics
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    const double s = wave_sum(match ? v[i] : 0.0);
-    if ((int)lane_id() == leader) atomic_add(base + lkey * STRIDE + i, s);
-  }
-  if (has && !match) {
+    for (int i = 0; i < NV; ++i) {
+      const double s = wave_sum(match ? v[i] : 0.0);
+      if ((int)lane_id() == leader) atomic_add(base + lkey * STRIDE + i, s);
+    }
+    if (has && !match) {
+#pragma unroll
+      for (int i = 0; i < NV; ++i) atomic_add(base + key * STRIDE + i, v[i]);
+    }
+  } else if (has) {
+    // keys all distinct so far: per-lane atomics, no reduction
 #pragma unroll
     for (int i = 0; i < NV; ++i) atomic_add(base + key * STRIDE + i, v[i]);
   }
@@ -694,10 +724,14 @@ struct Finish {
 __device__ bool pend_add(uint32_t* keys, double* vals, size_t stride, uint32_t npend,
                          uint32_t key, const double* w) {
   int slot_k = -1, free_k = -1;
-  for (uint32_t k = 0; k < npend; ++k) {
-    const uint32_t kk = keys[k * stride];
-    if (kk == key) { slot_k = (int)k; break; }
-    if (kk == NONE && free_k < 0) free_k = (int)k;
+  uint32_t kk[MAX_PEND];
+#pragma unroll
+  for (int k = 0; k < MAX_PEND; ++k) kk[k] = (uint32_t)k < npend ? keys[k * stride] : 0u;
+#pragma unroll
+  for (int k = MAX_PEND - 1; k >= 0; --k) {
+    if ((uint32_t)k >= npend) continue;
+    if (kk[k] == key) slot_k = k;
+    if (kk[k] == NONE) free_k = k;
   }
   if (slot_k < 0 && free_k >= 0) {
     slot_k = free_k;
@@ -718,7 +752,7 @@ __device__ void spill(double* base, uint64_t key, const double* w, unsigned long
   }
 }
 
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
 k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __restrict__ list,
         const unsigned* __restrict__ count_ptr, RayRec* __restrict__ rays,
         const HitRec* __restrict__ hits, uint8_t* __restrict__ emitted,
@@ -907,13 +941,13 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
           const uint32_t rkey = (uint32_t)is.receiver * 2u + (side == SSOL_FRONT_ ? 0u : 1u);
           if (!pend_add(P.pkey + slot, P.pval + slot, stride, prm.npend, rkey, w10)) {
             spill(T.rcv + (size_t)(blockIdx.x & T.mask_r) * T.nrcv * 40u, rkey, w10, &C->spills);
-            spill(T.sxr, (uint64_t)cur.w * S.nrcv * 2u + rkey, w10, &C->spills);
+            spill(T.sxr + (size_t)(blockIdx.x & T.mask_x) * T.nsamp * T.nrcv * 40u, (uint64_t)cur.w * S.nrcv * 2u + rkey, w10, &C->spills);
           }
           if (is.prim_tally >= 0) {
             const uint32_t pkey = (uint32_t)(((uint64_t)is.prim_tally + prim) * 2u +
                                              (side == SSOL_FRONT_ ? 0u : 1u));
             if (!pend_add(P.qkey + slot, P.qval + slot, stride, prm.npend, pkey, w10))
-              spill(T.prim, pkey, w10, &C->spills);
+              spill(T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u, pkey, w10, &C->spills);
             if (A.grad && A.p[slot] != NONE) A.s[slot].w += A.gmap[pkey] * w10[5];
           }
           if (A.grad && A.p[slot] != NONE) A.s[slot].w += A.d_total * (in0 - out0);
@@ -1006,7 +1040,7 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
     if (!has) for (int f = 0; f < 20; ++f) v[f] = 0;
     keyed_add<20>(has, has ? key : 0, v, rcv_base);
     const uint64_t xkey = has ? (uint64_t)fin.samp_slot * (uint64_t)S.nrcv * 2u + key : 0;
-    keyed_add<20>(has, xkey, v, T.sxr);
+    keyed_add<20>(has, xkey, v, T.sxr + (size_t)(blockIdx.x & T.mask_x) * T.nsamp * T.nrcv * 40u);
   }
   for (uint32_t k = 0; k < prm.npend; ++k) {
     uint32_t key = NONE;
@@ -1024,7 +1058,8 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
     }
     const bool has = done && key != NONE;
     if (has) {
-      for (int f = 0; f < 20; ++f) atomic_add(T.prim + (size_t)key * 20 + f, v[f]);
+      double* pb = T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u;
+      for (int f = 0; f < 20; ++f) atomic_add(pb + (size_t)key * 20 + f, v[f]);
     }
   }
 
@@ -1150,6 +1185,12 @@ __global__ void k_test_quadric(int quadric, double4 qp, uint32_t n, const double
     t[i] = INFINITY;
     nrm[3 * i] = nrm[3 * i + 1] = nrm[3 * i + 2] = 0;
   }
+}
+
+// One wave_sum per wave of 64 inputs (blocks of 64 threads: every lane active)
+__global__ void __launch_bounds__(64) k_test_wave_sum(const double* in, double* out) {
+  const double s = wave_sum(in[blockIdx.x * 64 + threadIdx.x]);
+  out[blockIdx.x * 64 + threadIdx.x] = s;
 }
 
 // Uniforms from an array (0.5 once it is exhausted), for fixed-input tests
@@ -1557,10 +1598,12 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     return (uint32_t)k;
   };
   const uint32_t sg = pow2_stripes((size_t)njobs * 16), sr = pow2_stripes((size_t)c->nrcv * 40),
-                 ss = pow2_stripes((size_t)c->nsamp * 5);
+                 ss = pow2_stripes((size_t)c->nsamp * 5),
+                 sx = pow2_stripes((size_t)c->nsamp * c->nrcv * 40),
+                 sp = pow2_stripes((size_t)c->nprim * 40);
   const size_t n_global = (size_t)sg * njobs * 16, n_rcv = (size_t)sr * c->nrcv * 40,
                n_samp = (size_t)ss * c->nsamp * 4;
-  const size_t n_sxr = (size_t)c->nsamp * c->nrcv * 40, n_prim = (size_t)c->nprim * 40;
+  const size_t n_sxr = (size_t)sx * c->nsamp * c->nrcv * 40, n_prim = (size_t)sp * c->nprim * 40;
   const size_t n_u64 = (size_t)ss * c->nsamp;
   const size_t tally_bytes = (n_global + n_rcv + n_samp + n_sxr + n_prim + n_u64) * 8 + 64;
   if ((r = dev_alloc(c->tallies, tally_bytes))) return r;
@@ -1575,6 +1618,9 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   c->T.mask_g = sg - 1;
   c->T.mask_r = sr - 1;
   c->T.mask_s = ss - 1;
+  c->T.mask_x = sx - 1;
+  c->T.mask_p = sp - 1;
+  c->T.nprim = c->nprim;
   c->T.nrcv = c->nrcv;
   c->T.nsamp = c->nsamp;
   c->T.njobs = njobs;
@@ -1737,6 +1783,8 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   if ((r = fold(c->T.rcv, (size_t)c->nrcv * 40, sr))) return r;
   if ((r = fold(c->T.samp, (size_t)c->nsamp * 4, ss))) return r;
   if ((r = fold(c->T.samp_count, (size_t)c->nsamp, ss))) return r;
+  if ((r = fold(c->T.sxr, (size_t)c->nsamp * c->nrcv * 40, sx))) return r;
+  if ((r = fold(c->T.prim, (size_t)c->nprim * 40, sp))) return r;
   float total_ms = 0;
   HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1939,6 +1987,22 @@ SSOLH_API int ssolh_test_threefry(ssolh_context* c, uint32_t n, const uint64_t* 
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(out, d_o, n * 32, hipMemcpyDeviceToHost));
   (void)hipFree(d_c); (void)hipFree(d_k); (void)hipFree(d_o);
+  return 0;
+}
+
+SSOLH_API int ssolh_test_wave_sum(ssolh_context* c, uint32_t nwaves, const double* in,
+                                  double* out) {
+  if (!c || !nwaves || !in || !out) return SSOLH_ERR_ARG;
+  HIP_TRY(hipSetDevice(c->device));
+  double *d_in, *d_out;
+  const size_t bytes = (size_t)nwaves * 64 * sizeof(double);
+  HIP_TRY(hipMalloc(&d_in, bytes)); HIP_TRY(hipMalloc(&d_out, bytes));
+  HIP_TRY(hipMemcpy(d_in, in, bytes, hipMemcpyHostToDevice));
+  k_test_wave_sum<<<nwaves, 64, 0, c->stream>>>(d_in, d_out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(out, d_out, bytes, hipMemcpyDeviceToHost));
+  (void)hipFree(d_in); (void)hipFree(d_out);
   return 0;
 }
 

@@ -411,10 +411,14 @@ k_generate32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams
 __device__ bool pend_add32(uint32_t* keys, float* vals, size_t stride, uint32_t npend,
                            uint32_t key, const float* w) {
   int slot_k = -1, free_k = -1;
-  for (uint32_t k = 0; k < npend; ++k) {
-    const uint32_t kk = keys[k * stride];
-    if (kk == key) { slot_k = (int)k; break; }
-    if (kk == NONE && free_k < 0) free_k = (int)k;
+  uint32_t kk[MAX_PEND];
+#pragma unroll
+  for (int k = 0; k < MAX_PEND; ++k) kk[k] = (uint32_t)k < npend ? keys[k * stride] : 0u;
+#pragma unroll
+  for (int k = MAX_PEND - 1; k >= 0; --k) {
+    if ((uint32_t)k >= npend) continue;
+    if (kk[k] == key) slot_k = k;
+    if (kk[k] == NONE) free_k = k;
   }
   if (slot_k < 0 && free_k >= 0) {
     slot_k = free_k;
@@ -597,13 +601,13 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
           const uint32_t rkey = (uint32_t)is.receiver * 2u + (side == SSOL_FRONT_ ? 0u : 1u);
           if (!pend_add32(P.pkey + slot, P.pval + slot, stride, prm.npend, rkey, w10)) {
             spill32(T.rcv + (size_t)(blockIdx.x & T.mask_r) * T.nrcv * 40u, rkey, w10, &C->spills);
-            spill32(T.sxr, (uint64_t)cur.w * S.nrcv * 2u + rkey, w10, &C->spills);
+            spill32(T.sxr + (size_t)(blockIdx.x & T.mask_x) * T.nsamp * T.nrcv * 40u, (uint64_t)cur.w * S.nrcv * 2u + rkey, w10, &C->spills);
           }
           if (is.prim_tally >= 0) {
             const uint32_t pkey = (uint32_t)(((uint64_t)is.prim_tally + prim) * 2u +
                                              (side == SSOL_FRONT_ ? 0u : 1u));
             if (!pend_add32(P.qkey + slot, P.qval + slot, stride, prm.npend, pkey, w10))
-              spill32(T.prim, pkey, w10, &C->spills);
+              spill32(T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u, pkey, w10, &C->spills);
           }
         } else {
           pc.z += in0 * kabs;
@@ -702,7 +706,7 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
     if (!has) for (int f = 0; f < 20; ++f) v[f] = 0;
     keyed_add<20>(has, has ? key : 0, v, rcv_base);
     const uint64_t xkey = has ? (uint64_t)fin.samp_slot * (uint64_t)S.nrcv * 2u + key : 0;
-    keyed_add<20>(has, xkey, v, T.sxr);
+    keyed_add<20>(has, xkey, v, T.sxr + (size_t)(blockIdx.x & T.mask_x) * T.nsamp * T.nrcv * 40u);
   }
   for (uint32_t k = 0; k < prm.npend; ++k) {
     uint32_t key = NONE;
@@ -720,7 +724,8 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
     }
     const bool has = done && key != NONE;
     if (has) {
-      for (int f = 0; f < 20; ++f) atomic_add(T.prim + (size_t)key * 20 + f, v[f]);
+      double* pb = T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u;
+      for (int f = 0; f < 20; ++f) atomic_add(pb + (size_t)key * 20 + f, v[f]);
     }
   }
 

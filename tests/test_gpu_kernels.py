@@ -214,3 +214,25 @@ def test_bsdf_samples_match_oracle(gpu, ctx, name, mat, eta):
         assert abs(w[i] - ww) <= 1e-12, (i, w[i], ww)
         assert lobe[i] == t, (i, lobe[i], t)
         assert np.allclose(wo[i], wwo, rtol=0, atol=1e-12), (i, wo[i], wwo)
+
+
+def test_wave_sum_lane_layout(gpu, ctx):
+    """The DPP wave reduction of the tallies: every lane returns the sum of
+    the 64 lanes (tolerance: a few ulp of the sum of magnitudes)."""
+    rng = np.random.default_rng(5)
+    nw = 64
+    x = rng.normal(size=(nw, 64))
+    x[0] = 0.0
+    x[1] = 1.0
+    x[2, :] = 0.0
+    x[2, 37] = 3.5          # a single lane carries the value
+    x[3] = np.arange(64)    # lane index: position errors show
+    out = np.zeros_like(x)
+    gpu.ssolh_test_wave_sum.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]
+    xc = np.ascontiguousarray(x)
+    assert gpu.ssolh_test_wave_sum(ctx, nw, _p(xc), _p(out)) == 0
+    want = x.sum(axis=1)
+    tol = 1e-14 * np.abs(x).sum(axis=1) + 1e-300
+    assert np.all(np.abs(out - want[:, None]) <= tol[:, None])
+    assert out[1, 0] == 64.0 and out[2, 5] == 3.5 and out[3, 63] == 2016.0
