@@ -348,6 +348,48 @@ struct ssolh_adjoint_args {
 SSOLH_API int ssolh_solve_adjoint
   (ssolh_context* ctx, const struct ssolh_adjoint_args* args, double* grad);
 
+/* Path recording (reference path tracker, src/ssol_solver.c:898-1127, kept
+ * per solve in src/ssol_estimator.c:270-322): every realisation records its
+ * vertices (start point, each hit, and the sun and escape points at the given
+ * lengths when they are > 0) with the flux carried toward each vertex, and
+ * its path type. FP64 form, host-driven iterations. */
+enum ssolh_path_type {
+  SSOLH_PATH_MISSING = 0,  /* same values as enum ssol_path_type */
+  SSOLH_PATH_SHADOW = 1,
+  SSOLH_PATH_SUCCESS = 2,
+  SSOLH_PATH_ERROR = 3,
+  SSOLH_PATH_DROPPED = 255 /* cancelled by Russian roulette, or never run */
+};
+
+struct ssolh_track_args {
+  double sun_ray_length;       /* > 0: vertex at start - length * sun dir */
+  double infinite_ray_length;  /* > 0: vertex at last point + length * dir */
+  uint64_t max_vertices;       /* capacity of the device vertex buffer */
+};
+
+struct ssolh_track_vertex {
+  uint64_t path;               /* realisation id (0 .. realisations-1) */
+  uint32_t index;              /* position of the vertex in its path */
+  uint32_t pad__;
+  double pos[3];
+  double weight;
+};
+
+/* Solve of `args` that records paths. *nvertices receives the number of
+ * vertices the paths produced; when it exceeds max_vertices the buffer kept
+ * only the first max_vertices appended, and the same solve run again with a
+ * larger buffer records them all (the realisations are the same). Tallies
+ * are those of the solve, as after ssolh_solve. */
+SSOLH_API int ssolh_solve_tracked
+  (ssolh_context* ctx, const struct ssolh_solve_args* args,
+   const struct ssolh_track_args* track, uint64_t* nvertices);
+
+/* Copy the recorded vertices (unordered, min(nvertices, max_vertices) of
+ * them) and the type of each realisation (npaths = realisations). */
+SSOLH_API int ssolh_track_download
+  (ssolh_context* ctx, struct ssolh_track_vertex* vertices, uint64_t nvertices,
+   uint8_t* types, uint64_t npaths);
+
 /* Batched solve (see ssolh_batch_args); out: [nsuns] */
 SSOLH_API int ssolh_solve_batch
   (ssolh_context* ctx, const struct ssolh_batch_args* args,

@@ -150,6 +150,23 @@ int oracle_solve
   (struct ooracle* o, uint64_t rng_key, uint64_t rng_counter,
    uint64_t realisations, uint64_t max_failed, int nthreads);
 
+/* Solve that records every path like the reference's path tracker
+ * (src/ssol_solver.c:898-1127; src/ssol_estimator.c:270-322), in one thread,
+ * paths in realisation order. A negative length takes the reference's
+ * default, 0.75 x the largest extent of the scene's box
+ * (src/ssol_solver.c:562-570, 1211-1220). Types are enum ssol_path_type
+ * values (0 missing, 1 shadow, 2 success, 3 error). */
+int oracle_solve_tracked
+  (struct ooracle* o, uint64_t rng_key, uint64_t rng_counter,
+   uint64_t realisations, uint64_t max_failed, double sun_ray_length,
+   double infinite_ray_length);
+size_t oracle_tracked_count(const struct ooracle* o);
+/* vertices: [nvertices][4] = x, y, z, weight; valid until the next solve */
+int oracle_tracked_path
+  (const struct ooracle* o, size_t ipath, int* type, size_t* nvertices,
+   const double** vertices);
+double oracle_tracked_extend(const struct ooracle* o);
+
 /* Results: E, V, SE triples like the reference getters */
 void oracle_get_global(struct ooracle* o, double out[6][3]);
 int oracle_get_receiver

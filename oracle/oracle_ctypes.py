@@ -98,6 +98,16 @@ def _declare(L):
     L.oracle_destroy.argtypes = [C.c_void_p]
     L.oracle_solve.restype = C.c_int
     L.oracle_solve.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]
+    L.oracle_solve_tracked.restype = C.c_int
+    L.oracle_solve_tracked.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64,
+                                       C.c_uint64, C.c_double, C.c_double]
+    L.oracle_tracked_count.restype = C.c_size_t
+    L.oracle_tracked_count.argtypes = [C.c_void_p]
+    L.oracle_tracked_path.restype = C.c_int
+    L.oracle_tracked_path.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_size_t), C.POINTER(C.POINTER(C.c_double))]
+    L.oracle_tracked_extend.restype = C.c_double
+    L.oracle_tracked_extend.argtypes = [C.c_void_p]
     L.oracle_get_global.argtypes = [C.c_void_p, (C.c_double * 3) * 6]
     L.oracle_get_receiver.argtypes = [C.c_void_p, C.c_size_t, C.c_int, R10x3]
     L.oracle_get_sampled.argtypes = [C.c_void_p, C.c_size_t, (C.c_double * 3) * 2,
@@ -263,6 +273,26 @@ class Oracle:
     def solve(self, n: int, seed: int = 0, counter: int = 0, max_failed: int = 0,
               nthreads: int = 0) -> int:
         return lib().oracle_solve(self._h, seed, counter, n, max_failed, nthreads)
+
+    def solve_tracked(self, n: int, seed: int = 0, counter: int = 0, max_failed: int = 0,
+                      sun_ray_length: float = -1.0, infinite_ray_length: float = -1.0) -> int:
+        """Single-threaded solve recording every path (reference path tracker)."""
+        return lib().oracle_solve_tracked(self._h, seed, counter, n, max_failed,
+                                          sun_ray_length, infinite_ray_length)
+
+    def tracked_paths(self):
+        """[(type, vertices [n][4] = x, y, z, weight)] in realisation order."""
+        out = []
+        for i in range(lib().oracle_tracked_count(self._h)):
+            ty, n, v = C.c_int(0), C.c_size_t(0), C.POINTER(C.c_double)()
+            assert lib().oracle_tracked_path(self._h, i, C.byref(ty), C.byref(n), C.byref(v)) == 0
+            a = np.ctypeslib.as_array(v, shape=(n.value * 4,)).reshape(n.value, 4).copy() \
+                if n.value else np.zeros((0, 4))
+            out.append((ty.value, a))
+        return out
+
+    def tracked_extend(self) -> float:
+        return lib().oracle_tracked_extend(self._h)
 
     @staticmethod
     def _res(a):

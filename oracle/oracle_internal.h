@@ -203,6 +203,15 @@ struct ooracle {
   uint64_t* nb_samples;      /* [nsamp] */
   struct omc* sxr;           /* [nsamp][nrcv][2][10] or NULL */
   struct omc* prim;          /* [nprim_slots][2][10] */
+  /* path recording (src/ssol_solver.c:898-1127), single-threaded solves */
+  int track;
+  double track_sun_len, track_inf_len;
+  double* tv;                /* [n][4]: position, weight */
+  size_t tv_n, tv_cap;
+  size_t* tp_first;          /* per recorded path: first vertex, count, type */
+  size_t* tp_count;
+  int* tp_type;
+  size_t tp_n, tp_cap;
 };
 
 int oscene_build(struct ooracle* o);

@@ -25,7 +25,48 @@ struct otctx {
   struct omc** touched;
   size_t ntouched, cap;
   uint64_t realisations, segments;
+  /* path recording: type of the last path, index of its first vertex */
+  int path_type;
+  size_t path_first;
 };
+
+/* Path types (enum ssol_path_type, src/ssol.h:70-75); -1: not registered */
+enum { P_MISSING, P_SHADOW, P_SUCCESS, P_ERROR, P_DROPPED = -1 };
+
+/* path_add_vertex (src/ssol_solver.c:588-600) */
+static void track_vertex(struct ooracle* o, const double pos[3], double w)
+{
+  if(o->tv_n == o->tv_cap) {
+    o->tv_cap = o->tv_cap ? 2 * o->tv_cap : 1024;
+    o->tv = realloc(o->tv, o->tv_cap * 4 * sizeof(double));
+  }
+  o->tv[4*o->tv_n + 0] = pos[0];
+  o->tv[4*o->tv_n + 1] = pos[1];
+  o->tv[4*o->tv_n + 2] = pos[2];
+  o->tv[4*o->tv_n + 3] = w;
+  o->tv_n++;
+}
+
+/* path_register_and_clear (src/ssol_solver.c:572-588); dropped paths
+ * (roulette) give their vertices back */
+static void track_finish(struct ooracle* o, struct otctx* c, int r)
+{
+  const int type = r ? P_ERROR : c->path_type;
+  if(type == P_DROPPED) {
+    o->tv_n = c->path_first;
+    return;
+  }
+  if(o->tp_n == o->tp_cap) {
+    o->tp_cap = o->tp_cap ? 2 * o->tp_cap : 1024;
+    o->tp_first = realloc(o->tp_first, o->tp_cap * sizeof(size_t));
+    o->tp_count = realloc(o->tp_count, o->tp_cap * sizeof(size_t));
+    o->tp_type = realloc(o->tp_type, o->tp_cap * sizeof(int));
+  }
+  o->tp_first[o->tp_n] = c->path_first;
+  o->tp_count[o->tp_n] = o->tv_n - c->path_first;
+  o->tp_type[o->tp_n] = type;
+  o->tp_n++;
+}
 
 static void omc_flush(struct omc* d)
 {
@@ -232,6 +273,8 @@ trace_path(struct ooracle* o, struct otctx* c, uint64_t ir, uint64_t key,
 
   memset(&p, 0, sizeof(p));
   c->ntouched = 0;
+  c->path_first = o->tv_n;
+  c->path_type = P_MISSING;
   orng_init(&rng, key, ctr0 + ir);
 
   /* Area-weighted start point on the sampling view (src/ssol_solver.c:237-252) */
@@ -300,6 +343,15 @@ trace_path(struct ooracle* o, struct otctx* c, uint64_t ir, uint64_t key,
 
   /* Starting medium (src/ssol_solver.c:307-323) */
   p.mtl = point_material(o, &p);
+  if(o->track) {
+    /* sun point and start point (src/ssol_solver.c:909-921) */
+    if(o->track_sun_len > 0) {
+      double sp[3];
+      for(i = 0; i < 3; ++i) sp[i] = p.pos[i] - p.dir[i] * o->track_sun_len;
+      track_vertex(o, sp, s->sun.dni);
+    }
+    track_vertex(o, p.pos, p.initial_flux);
+  }
   if(p.mtl->type == OM_DIELECTRIC || p.mtl->type == OM_THIN) {
     in_med = p.side == OSIDE_FRONT ? &p.mtl->medium_in : &p.mtl->medium_out;
   } else {
@@ -323,7 +375,9 @@ trace_path(struct ooracle* o, struct otctx* c, uint64_t ir, uint64_t key,
   if(!is_lit) {
     omc_add(c, c->samp + 2*p.samp_slot + 1, ir, p.initial_flux);
     omc_add(c, &c->global[G_SHADOWED], ir, p.initial_flux);
+    c->path_type = P_SHADOW;
   } else {
+    int hit_a_receiver = 0;
     ov_set(org, p.pos);
     ov_set(rdir, p.dir);
     hit_t = 0;
@@ -351,6 +405,7 @@ trace_path(struct ooracle* o, struct otctx* c, uint64_t ir, uint64_t key,
       }
 
       if(hit_receiver) {
+        hit_a_receiver = 1;
         update_mc(o, c, &p, ir);
       } else {
         omc_add(c, &c->global[G_OTHER], ir, p.in[0] * p.kabs);
@@ -377,6 +432,12 @@ trace_path(struct ooracle* o, struct otctx* c, uint64_t ir, uint64_t key,
         otrace(o, &ray);
         c->segments++;
         if(!ray.hit) {
+          /* escape point (src/ssol_solver.c:1012-1019) */
+          if(o->track && o->track_inf_len > 0) {
+            double ep[3];
+            for(i = 0; i < 3; ++i) ep[i] = p.pos[i] + rdir[i] * o->track_inf_len;
+            track_vertex(o, ep, p.out[0]);
+          }
           last_segment = 1;
           if(!omedia_ceq(s, out_med, &o->air)) return 5;
         }
@@ -425,10 +486,15 @@ trace_path(struct ooracle* o, struct otctx* c, uint64_t ir, uint64_t key,
         for(i = 0; i < 3; ++i) p.N[i] = -p.N[i];
       }
       p.mtl = point_material(o, &p);
+      if(o->track) track_vertex(o, p.pos, p.out[0]); /* src/ssol_solver.c:1080-1083 */
       in_med = out_med;
     }
-    if(killed) return 0;
+    if(killed) {
+      c->path_type = P_DROPPED;
+      return 0;
+    }
     omc_add(c, &c->global[G_MISSING], ir, p.out[0]);
+    c->path_type = hit_a_receiver ? P_SUCCESS : P_MISSING;
   }
   omc_add(c, c->samp + 2*p.samp_slot + 0, ir, p.cos_factor);
   omc_add(c, &c->global[G_COS], ir, p.cos_factor);
@@ -472,10 +538,18 @@ static void free_results(struct ooracle* o)
   o->nb_samples = NULL;
 }
 
+static void free_tracked(struct ooracle* o)
+{
+  free(o->tv); free(o->tp_first); free(o->tp_count); free(o->tp_type);
+  o->tv = NULL; o->tp_first = o->tp_count = NULL; o->tp_type = NULL;
+  o->tv_n = o->tv_cap = o->tp_n = o->tp_cap = 0;
+}
+
 void
 oracle_destroy(struct ooracle* o)
 {
   if(!o) return;
+  free_tracked(o);
   free_results(o);
   oscene_release(o);
   free(o);
@@ -519,6 +593,8 @@ oracle_solve
 
   if(!realisations) return 1;
   if(nthreads <= 0) nthreads = omp_get_num_procs();
+  if(o->track) nthreads = 1; /* paths are appended in realisation order */
+  else free_tracked(o);       /* recorded paths belong to the last solve */
   free_results(o);
   ctxs = calloc((size_t)nthreads, sizeof(*ctxs));
   if(!ctxs) return 2;
@@ -530,6 +606,7 @@ oracle_solve
     int r;
     if(abort_flag) continue;
     r = trace_path(o, c, (uint64_t)i, rng_key, rng_counter);
+    if(o->track) track_finish(o, c, r);
     if(r) {
       uint64_t nf;
       omc_apply_factor(c, (uint64_t)i, 0);
@@ -670,3 +747,60 @@ unsigned oracle_shape_rt_vertex_count(const struct ooracle* o, size_t s)
 { return o->shapes[s].nverts; }
 void oracle_shape_rt_vertex(const struct ooracle* o, size_t s, unsigned i, double pos[3])
 { ov_set(pos, o->shapes[s].rt_verts + 3*i); }
+
+/* compute_infinite_path_segment_extend (src/ssol_solver.c:562-570) over the
+ * world-space ray-traced triangles */
+double
+oracle_tracked_extend(const struct ooracle* o)
+{
+  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  double e = 0;
+  size_t t;
+  int k, v;
+  for(t = 0; t < o->ntris; ++t) {
+    const struct owtri* tr = o->tris + t;
+    for(v = 0; v < 3; ++v) {
+      for(k = 0; k < 3; ++k) {
+        const double x = tr->v0[k] + (v == 1 ? tr->e1[k] : 0) + (v == 2 ? tr->e2[k] : 0);
+        if(x < lo[k]) lo[k] = x;
+        if(x > hi[k]) hi[k] = x;
+      }
+    }
+  }
+  for(k = 0; k < 3; ++k) if(hi[k] >= lo[k] && hi[k] - lo[k] > e) e = hi[k] - lo[k];
+  return e * 0.75;
+}
+
+int
+oracle_solve_tracked
+  (struct ooracle* o, uint64_t rng_key, uint64_t rng_counter,
+   uint64_t realisations, uint64_t max_failed, double sun_ray_length,
+   double infinite_ray_length)
+{
+  int res;
+  free_tracked(o);
+  if(sun_ray_length < 0 || infinite_ray_length < 0) {
+    const double ext = oracle_tracked_extend(o);
+    if(sun_ray_length < 0) sun_ray_length = ext;
+    if(infinite_ray_length < 0) infinite_ray_length = ext;
+  }
+  o->track = 1;
+  o->track_sun_len = sun_ray_length;
+  o->track_inf_len = infinite_ray_length;
+  res = oracle_solve(o, rng_key, rng_counter, realisations, max_failed, 1);
+  o->track = 0;
+  return res;
+}
+
+size_t oracle_tracked_count(const struct ooracle* o) { return o->tp_n; }
+
+int
+oracle_tracked_path
+  (const struct ooracle* o, size_t i, int* type, size_t* n, const double** v)
+{
+  if(i >= o->tp_n) return 1;
+  *type = o->tp_type[i];
+  *n = o->tp_count[i];
+  *v = o->tv + 4 * o->tp_first[i];
+  return 0;
+}

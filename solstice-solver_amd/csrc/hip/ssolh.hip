@@ -142,7 +142,8 @@ enum : uint32_t {
   PF_STAGE_BOUNCE = 1u << 0,   // else: waiting for the shadow ray
   PF_HIT_VIRTUAL = 1u << 1,    // the point that emitted the ray is virtual
   PF_IN_ATM = 1u << 2,         // the incoming segment of that point was in air
-  PF_ACTIVE = 1u << 3
+  PF_ACTIVE = 1u << 3,
+  PF_HIT_RECEIVER = 1u << 4    // the path reached a receiver (path tracking)
 };
 
 struct PathState {
@@ -228,6 +229,32 @@ struct AdjParams {
   double4* s;                // [slots]: scores of the path (aim 1, aim 2, slope), q
   uint32_t* p;               // [slots]: parameter slot of the path or NONE
 };
+
+// Path recording (src/ssol_solver.c:898-1127): null vtx <=> off. Vertices
+// are appended in arrival order, tagged with their realisation and index.
+struct TrackParams {
+  ssolh_track_vertex* vtx;
+  unsigned long long* nvtx;  // vertices produced (may exceed cap)
+  unsigned long long cap;
+  uint8_t* type;             // [N] SSOLH_PATH_* of each realisation
+  uint32_t* seq;             // [slots] vertices recorded by the slot's path
+  double sun_len, inf_len;
+};
+
+__device__ void track_vertex(const TrackParams& K, uint32_t slot, uint64_t path, d3 p, double w) {
+  const uint32_t s = K.seq[slot];
+  K.seq[slot] = s + 1;
+  const unsigned long long i = atomicAdd(K.nvtx, 1ull);
+  if (i < K.cap) {
+    ssolh_track_vertex v;
+    v.path = path;
+    v.index = s;
+    v.pad__ = 0;
+    v.pos[0] = p.x; v.pos[1] = p.y; v.pos[2] = p.z;
+    v.weight = w;
+    K.vtx[i] = v;
+  }
+}
 
 // Work list: slot of entry q (a null list is the identity)
 __device__ __forceinline__ uint32_t slot_of(const uint32_t* list, uint32_t q) {
@@ -589,7 +616,7 @@ __device__ uint32_t start_medium(const DScene& S, const ssolh_material& m, uint3
 
 __device__ void generate(const DScene& S, const PathState& P, const SolveParams& prm,
                          uint32_t slot, uint64_t path, Emit& e, int& samp_slot,
-                         const AdjParams& A) {
+                         const AdjParams& A, const TrackParams& K) {
   PathRng rng;
   rng.init(prm.key, prm.counter + path, 0);
   // area-weighted primitive through the alias table, uniform point
@@ -639,6 +666,13 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
 
   const ssolh_material& m = S.mtls[side == SSOL_FRONT_ ? is.mtl_front : is.mtl_back];
   const uint32_t med = start_medium(S, m, side);
+
+  if (K.vtx) {
+    // sun point and start point (src/ssol_solver.c:909-921)
+    K.seq[slot] = 0;
+    if (K.sun_len > 0) track_vertex(K, slot, path, pos - dir * K.sun_len, sun.dni);
+    track_vertex(K, slot, path, pos, w0);
+  }
 
   if (A.grad) {
     // w0 is proportional to |N.w|: d log w0 / d angle = ((a x N).w) / (N.w)
@@ -695,13 +729,13 @@ __device__ void flush_sampled_count(bool has, int samp_slot, const Tallies& T) {
 // runs realisations s + k * slots, k = 1, 2, ... while they are below N.
 __global__ void __launch_bounds__(BLOCK)
 k_generate(DScene S, PathState P, SolveParams prm, Tallies T, RayRec* __restrict__ rays,
-           uint32_t count, AdjParams A) {
+           uint32_t count, AdjParams A, TrackParams K) {
   const uint32_t slot = blockIdx.x * BLOCK + threadIdx.x;
   const bool live = slot < count;
   Emit e;
   e.has = false;
   int samp_slot = 0;
-  if (live) generate(S, P, prm, slot, slot, e, samp_slot, A);
+  if (live) generate(S, P, prm, slot, slot, e, samp_slot, A, K);
   flush_sampled_count(live, samp_slot, T);
   if (e.has) rays[slot] = e.r;
 }
@@ -756,7 +790,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
 k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __restrict__ list,
         const unsigned* __restrict__ count_ptr, RayRec* __restrict__ rays,
         const HitRec* __restrict__ hits, uint8_t* __restrict__ emitted,
-        uint32_t* __restrict__ block_emits, Counters* C, AdjParams A) {
+        uint32_t* __restrict__ block_emits, Counters* C, AdjParams A, TrackParams K) {
   const uint32_t count = *count_ptr;
   if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: block_count syncs
   const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
@@ -821,6 +855,9 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
       bool last = false;
       if (!found) {
         last = true;
+        // escape point (src/ssol_solver.c:1012-1019)
+        if (K.vtx && K.inf_len > 0)
+          track_vertex(K, slot, path, pos + mk(r.dx, r.dy, r.dz) * K.inf_len, oi.x);
         if (out_med != S.air) fin.kind = FIN_FAILED;  // ray lost outside air
       }
       if (fin.kind == FIN_NONE && (last || !hit_virtual)) {
@@ -858,6 +895,7 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
         in_med = out_med;
         hit_t = h.t;
         head = true;
+        if (K.vtx) track_vertex(K, slot, path, pos, oi.x);  // src/ssol_solver.c:1080-1083
       }
     }
 
@@ -868,6 +906,7 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
       const bool in_atm = in_med == S.air;
       const bool receiver = (is.receiver_mask & side) != 0;
       const bool virt = m.type == SSOLH_MTL_VIRTUAL;
+      if (receiver) flags |= PF_HIT_RECEIVER;
       double trans = 1;
       if (hit_t > 0) {
         const double k = data_value(S.spectra, S.media[in_med].extinction, wl);
@@ -974,13 +1013,23 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
           e.r.iss_from = iss;
           e.r.prim_from = prim;
           e.r.flags = side << RF_SIDE_SHIFT;
-          flags = PF_ACTIVE | PF_STAGE_BOUNCE | (virt ? PF_HIT_VIRTUAL : 0u) | (in_atm ? PF_IN_ATM : 0u);
+          flags = (flags & PF_HIT_RECEIVER) | PF_ACTIVE | PF_STAGE_BOUNCE |
+                  (virt ? PF_HIT_VIRTUAL : 0u) | (in_atm ? PF_IN_ATM : 0u);
           oi = make_double4(out0, out1, out2, in0);
           out_med = next_med;
         }
       }
     }
 
+    if (K.vtx && fin.kind != FIN_NONE) {
+      // path type (src/ssol_solver.c:931, 1092-1094, 1111-1125)
+      uint8_t ty;
+      if (fin.kind == FIN_KILLED) ty = SSOLH_PATH_DROPPED;
+      else if (fin.kind == FIN_FAILED) ty = SSOLH_PATH_ERROR;
+      else if (!(meta.w & PF_STAGE_BOUNCE) && h.iss != NONE) ty = SSOLH_PATH_SHADOW;
+      else ty = (flags & PF_HIT_RECEIVER) ? SSOLH_PATH_SUCCESS : SSOLH_PATH_MISSING;
+      K.type[path] = ty;
+    }
     if (fin.kind == FIN_DONE) {
       fin.g[0] = dc.w;
       fin.g[1] = pd.x;
@@ -1081,7 +1130,7 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
   int samp_slot = 0;
   bool gen = false;
   if (want && id < prm.N) {
-    generate(S, P, prm, slot, id, e, samp_slot, A);
+    generate(S, P, prm, slot, id, e, samp_slot, A, K);
     gen = true;
   } else if (live && fin.kind != FIN_NONE) {
     P.meta[slot] = make_uint4(0, 0, 0, 0);
@@ -1263,6 +1312,10 @@ struct ssolh_context {
   // solve buffers
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, counters, tallies, jobs;
   DevBuf adj_state, adj_in, adj_grad;
+  DevBuf trk_vtx, trk_type, trk_seq, trk_count;
+  uint64_t trk_paths = 0;           // realisations of the last tracked solve
+  unsigned long long trk_nvtx = 0;  // vertices it produced
+  uint64_t trk_cap = 0;
   uint32_t ninstances = 0;
   uint32_t njobs = 1;
   PathState P{};
@@ -1394,7 +1447,8 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
   free_scene(c);
   for (DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
                     &c->block_emits, &c->block_offsets, &c->counters, &c->tallies, &c->jobs,
-                    &c->adj_state, &c->adj_in, &c->adj_grad})
+                    &c->adj_state, &c->adj_in, &c->adj_grad, &c->trk_vtx, &c->trk_type,
+                    &c->trk_seq, &c->trk_count})
     if (b->p) (void)hipFree(b->p);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1481,6 +1535,11 @@ namespace {
 // many iterations (rounded up to even) into a HIP graph replayed until the
 // work list is empty; 0 runs the iterations from the host one by one.
 // Host description of an adjoint solve (host pointers)
+struct TrackHost {
+  double sun_len, inf_len;
+  uint64_t cap;
+};
+
 struct AdjHost {
   uint32_t nparams;
   const int32_t* inst_param;  // [ninstances]
@@ -1492,8 +1551,14 @@ struct AdjHost {
 int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, uint64_t counter,
               bool fp32, uint32_t path_slots, uint32_t npend, uint32_t njobs,
               const uint64_t* job_first_h, const ssolh_sun* job_suns_h, uint32_t graph_iters,
-              const AdjHost* adj = nullptr) {
+              const AdjHost* adj = nullptr, const TrackHost* trk = nullptr) {
   if (adj && fp32) { set_err("ssolh_solve_adjoint: the adjoint runs in the FP64 form"); return SSOLH_ERR_ARG; }
+  if (trk && (fp32 || adj || job_suns_h)) {
+    set_err("ssolh_solve_tracked: paths are recorded by plain FP64 solves only");
+    return SSOLH_ERR_ARG;
+  }
+  if (trk) graph_iters = 0;
+  else c->trk_paths = 0;  // the recorded paths belong to the last solve only
   if (npend > MAX_PEND) { set_err("ssolh_solve: at most %d pending entries", MAX_PEND); return SSOLH_ERR_LIMIT; }
   uint64_t slots64 = path_slots ? path_slots : (1u << 21);
   if (slots64 > N) slots64 = N;
@@ -1581,6 +1646,26 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     A.d_total = adj->d_total;
     A.s = (double4*)c->adj_state.p;
     A.p = (uint32_t*)((char*)c->adj_state.p + (n * sizeof(double4) + 255) / 256 * 256);
+  }
+  // path recording buffers
+  TrackParams K{};
+  if (trk) {
+    const uint64_t cap = std::max<uint64_t>(trk->cap, 1);
+    if ((r = dev_alloc(c->trk_vtx, cap * sizeof(ssolh_track_vertex)))) return r;
+    if ((r = dev_alloc(c->trk_type, N))) return r;
+    if ((r = dev_alloc(c->trk_seq, n * sizeof(uint32_t)))) return r;
+    if ((r = dev_alloc(c->trk_count, sizeof(unsigned long long)))) return r;
+    HIP_TRY(hipMemsetAsync(c->trk_type.p, SSOLH_PATH_DROPPED, N, c->stream));
+    HIP_TRY(hipMemsetAsync(c->trk_count.p, 0, sizeof(unsigned long long), c->stream));
+    K.vtx = (ssolh_track_vertex*)c->trk_vtx.p;
+    K.nvtx = (unsigned long long*)c->trk_count.p;
+    K.cap = cap;
+    K.type = (uint8_t*)c->trk_type.p;
+    K.seq = (uint32_t*)c->trk_seq.p;
+    K.sun_len = trk->sun_len;
+    K.inf_len = trk->inf_len;
+    c->trk_cap = cap;
+    c->trk_paths = N;
   }
   // jobs
   if (job_suns_h) {
@@ -1683,7 +1768,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     else
       k_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
                                                (RayRec*)c->rays.p, (const HitRec*)c->hits.p,
-                                               emitted, block_emits, ctr, A);
+                                               emitted, block_emits, ctr, A, K);
     HIP_TRY(hipGetLastError());
     if (!graph_iters) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
     k_scan<<<1, SCAN_THREADS, 0, c->stream>>>(block_emits, count, block_offsets, cnt + (i ^ 1),
@@ -1705,7 +1790,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
         c->scn, c->shapeq, c->P32, prm, c->T, (RayRec32*)c->rays.p, count);
   else
     k_generate<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
-        c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A);
+        c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A, K);
   HIP_TRY(hipGetLastError());
   uint32_t iter = 0;
   if (!graph_iters) {
@@ -1791,6 +1876,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   (void)hipEventElapsedTime(&total_ms, c->ev[0], c->ev[3]);
   HIP_TRY(hipMemcpy(&c->last, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
   c->segments = c->last.segments;
+  if (trk) HIP_TRY(hipMemcpy(&c->trk_nvtx, c->trk_count.p, sizeof(c->trk_nvtx), hipMemcpyDeviceToHost));
   c->kernel_ms = total_ms;
   c->trace_ms = ms_trace;
   c->shade_ms = ms_shade;
@@ -1878,6 +1964,51 @@ SSOLH_API int ssolh_solve_adjoint(ssolh_context* c, const struct ssolh_adjoint_a
   double n = 0;
   HIP_TRY(hipMemcpy(&n, c->T.global + 12, sizeof(double), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < (size_t)a->nparams * 3; ++i) grad[i] = n > 0 ? grad[i] / n : 0.0;
+  return 0;
+}
+
+SSOLH_API int ssolh_solve_tracked(ssolh_context* c, const struct ssolh_solve_args* a,
+                                  const struct ssolh_track_args* t, uint64_t* nvertices) {
+  if (!c || !a || !t || !nvertices || !a->realisations) {
+    set_err("ssolh_solve_tracked: invalid arguments");
+    return SSOLH_ERR_ARG;
+  }
+  if (!c->has_scene) { set_err("ssolh_solve_tracked: no scene uploaded"); return SSOLH_ERR_ARG; }
+  if (a->precision != SSOLH_FP64) {
+    set_err("ssolh_solve_tracked: paths are recorded by the FP64 form");
+    return SSOLH_ERR_ARG;
+  }
+  if (!(t->sun_ray_length == t->sun_ray_length) || !(t->infinite_ray_length == t->infinite_ray_length)) {
+    set_err("ssolh_solve_tracked: NaN ray length");
+    return SSOLH_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  TrackHost h{t->sun_ray_length, t->infinite_ray_length, t->max_vertices};
+  const uint32_t npend = a->pending_entries ? a->pending_entries : 4;
+  int r = run_solve(c, a->realisations, a->max_failed, a->rng_key, a->rng_counter, false,
+                    a->path_slots, npend, 1, nullptr, nullptr, 0, nullptr, &h);
+  if (r) return r;
+  *nvertices = c->trk_nvtx;
+  return 0;
+}
+
+SSOLH_API int ssolh_track_download(ssolh_context* c, struct ssolh_track_vertex* v, uint64_t nv,
+                                   uint8_t* types, uint64_t npaths) {
+  if (!c || (nv && !v) || (npaths && !types)) {
+    set_err("ssolh_track_download: invalid arguments");
+    return SSOLH_ERR_ARG;
+  }
+  if (!c->trk_paths) { set_err("ssolh_track_download: no tracked solve"); return SSOLH_ERR_ARG; }
+  if (nv > std::min<uint64_t>(c->trk_nvtx, c->trk_cap) || npaths > c->trk_paths) {
+    set_err("ssolh_track_download: asked for %llu vertices and %llu paths, the solve holds %llu and %llu",
+            (unsigned long long)nv, (unsigned long long)npaths,
+            (unsigned long long)std::min<uint64_t>(c->trk_nvtx, c->trk_cap),
+            (unsigned long long)c->trk_paths);
+    return SSOLH_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  if (nv) HIP_TRY(hipMemcpy(v, c->trk_vtx.p, nv * sizeof(ssolh_track_vertex), hipMemcpyDeviceToHost));
+  if (npaths) HIP_TRY(hipMemcpy(types, c->trk_type.p, npaths, hipMemcpyDeviceToHost));
   return 0;
 }
 

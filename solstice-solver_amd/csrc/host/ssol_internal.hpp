@@ -170,6 +170,11 @@ struct ssol_estimator : ssol_refcounted {
   std::vector<std::unique_ptr<rcv_ref>> refs;
   struct shape_ref { ssol_estimator* est; size_t first; const ssol_shape* shape; };
   std::vector<std::unique_ptr<shape_ref>> shape_refs;
+  // paths recorded by a solve with a path tracker, in realisation order;
+  // ssol_path::path__ points to one of `paths`
+  struct tracked_path { int type; const ssol_path_vertex* v; size_t n; };
+  std::vector<tracked_path> paths;
+  std::vector<ssol_path_vertex> path_vertices;
 };
 
 namespace ssol {

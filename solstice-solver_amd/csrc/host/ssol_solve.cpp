@@ -735,6 +735,94 @@ res_T check_sun(ssol_scene* scn, const char* who) {
   return RES_OK;
 }
 
+// Largest extent of the world box of the ray-traced triangles, times 0.75:
+// the default length of the tracked sun and escape segments
+// (compute_infinite_path_segment_extend, src/ssol_solver.c:562-570).
+double tracked_extend(const ssol_scene* scn) {
+  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (const ssol_instance* in : scn->instances) {
+    if (!in->object) continue;
+    const double* t = in->transform;
+    for (const auto& ss : in->object->sshapes) {
+      const std::vector<float>& pos = ss.shape->pos;
+      for (size_t v = 0; v + 2 < pos.size(); v += 3) {
+        for (int k = 0; k < 3; ++k) {
+          const double w = t[k] * pos[v] + t[3 + k] * pos[v + 1] + t[6 + k] * pos[v + 2] + t[9 + k];
+          lo[k] = std::min(lo[k], w);
+          hi[k] = std::max(hi[k], w);
+        }
+      }
+    }
+  }
+  double e = 0;
+  for (int k = 0; k < 3; ++k) if (hi[k] >= lo[k]) e = std::max(e, hi[k] - lo[k]);
+  return e * 0.75;
+}
+
+// Solve that records every path (FP64 form): runs once with a buffer sized
+// for a few vertices per path, and again with the exact size if the paths
+// produced more (the realisations are the same), then orders the vertices
+// by realisation and index. Paths cancelled by the roulette are dropped as
+// in src/ssol_solver.c:1111-1117.
+res_T solve_tracked(ssol_scene* scn, const ssolh_solve_args& a0, const ssol_path_tracker* tracker,
+                    ssol_estimator* est) {
+  ssol_device* dev = scn->dev;
+  double sun_len = tracker->sun_ray_length, inf_len = tracker->infinite_ray_length;
+  if (sun_len < 0 || inf_len < 0) {
+    const double ext = tracked_extend(scn);
+    if (sun_len < 0) sun_len = ext;
+    if (inf_len < 0) inf_len = ext;
+  }
+  ssolh_solve_args a = a0;
+  a.precision = SSOLH_FP64;
+  a.graph_iterations = 0;
+  ssolh_track_args t{sun_len, inf_len, 0};
+  t.max_vertices = std::max<uint64_t>(4096, a.realisations * 8);
+  uint64_t nv = 0;
+  int e = ssolh_solve_tracked(dev->gpu, &a, &t, &nv);
+  if (!e && nv > t.max_vertices) {
+    t.max_vertices = nv;
+    e = ssolh_solve_tracked(dev->gpu, &a, &t, &nv);
+  }
+  if (e) {
+    fprintf(stderr, "ssol_solve: GPU solve failed: %s\n", ssolh_last_error());
+    return e == SSOLH_ERR_ARG ? RES_BAD_ARG : RES_UNKNOWN_ERR;
+  }
+  std::vector<ssolh_track_vertex> v(nv);
+  std::vector<uint8_t> types(a.realisations);
+  e = ssolh_track_download(dev->gpu, v.data(), nv, types.data(), a.realisations);
+  if (e) {
+    fprintf(stderr, "ssol_solve: path download failed: %s\n", ssolh_last_error());
+    return RES_UNKNOWN_ERR;
+  }
+  std::sort(v.begin(), v.end(), [](const ssolh_track_vertex& x, const ssolh_track_vertex& y) {
+    return x.path != y.path ? x.path < y.path : x.index < y.index;
+  });
+  size_t nkept = 0, npaths = 0;
+  for (const auto& x : v) nkept += types[x.path] != SSOLH_PATH_DROPPED;
+  for (uint8_t ty : types) npaths += ty != SSOLH_PATH_DROPPED;
+  est->path_vertices.reserve(nkept);
+  est->paths.reserve(npaths);
+  std::vector<size_t> at;  // first vertex of each kept path
+  at.reserve(npaths);
+  size_t i = 0;
+  for (uint64_t p = 0; p < a.realisations; ++p) {
+    const size_t first = i;
+    while (i < v.size() && v[i].path == p) ++i;
+    if (types[p] == SSOLH_PATH_DROPPED) continue;
+    at.push_back(est->path_vertices.size());
+    for (size_t k = first; k < i; ++k) {
+      ssol_path_vertex pv;
+      for (int c = 0; c < 3; ++c) pv.pos[c] = v[k].pos[c];
+      pv.weight = v[k].weight;
+      est->path_vertices.push_back(pv);
+    }
+    est->paths.push_back({(int)types[p], nullptr, i - first});
+  }
+  for (size_t k = 0; k < est->paths.size(); ++k) est->paths[k].v = est->path_vertices.data() + at[k];
+  return RES_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -747,10 +835,6 @@ res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t
   if (N > (size_t)INT64_MAX || max_failed > (size_t)INT64_MAX) return RES_BAD_ARG;
   ssol_device* dev = scn->dev;
   if (res_T rs = check_sun(scn, "ssol_solve")) return rs;
-  if (tracker) {
-    log_error(dev, "ssol_solve: path tracking is not available on the GPU solver.\n");
-    return RES_BAD_ARG;
-  }
   res_T pr = prepare_scene(scn);
   if (pr != RES_OK) return pr;
   Flat& f = dev->cache->flat;
@@ -765,15 +849,23 @@ res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t
   a.path_slots = (uint32_t)std::min<size_t>(dev->path_slots, UINT32_MAX);
   a.pending_entries = 4;
   a.graph_iterations = dev->graph_iterations;
-  e = ssolh_solve(dev->gpu, &a);
-  if (e) {
-    fprintf(stderr, "ssol_solve: GPU solve failed: %s\n", ssolh_last_error());
-    return e == SSOLH_ERR_ARG ? RES_BAD_ARG : RES_UNKNOWN_ERR;
-  }
-
   ssol_estimator* est = new ssol_estimator();
   est->dev = dev;
   ref_get(dev);
+  if (tracker) {
+    const res_T tr = solve_tracked(scn, a, tracker, est);
+    if (tr != RES_OK) {
+      estimator_release(est);
+      return tr;
+    }
+  } else {
+    e = ssolh_solve(dev->gpu, &a);
+    if (e) {
+      fprintf(stderr, "ssol_solve: GPU solve failed: %s\n", ssolh_last_error());
+      estimator_release(est);
+      return e == SSOLH_ERR_ARG ? RES_BAD_ARG : RES_UNKNOWN_ERR;
+    }
+  }
   est->nrcv = L.nrcv;
   est->nsamp = L.nsamp;
   est->rcv.assign(L.nrcv * 40, 0);
@@ -926,31 +1018,38 @@ res_T ssol_estimator_get_rng_state(const struct ssol_estimator* e, const struct 
   return RES_OK;
 }
 
+// Tracked paths (src/ssol_estimator.c:270-322)
 res_T ssol_estimator_get_tracked_paths_count(const struct ssol_estimator* e, size_t* n) {
   if (!e || !n) return RES_BAD_ARG;
-  *n = 0;
+  *n = e->paths.size();
   return RES_OK;
 }
 
-res_T ssol_estimator_get_tracked_path(const struct ssol_estimator* e, const size_t,
+res_T ssol_estimator_get_tracked_path(const struct ssol_estimator* e, const size_t i,
                                       struct ssol_path* p) {
-  (void)e; (void)p;
-  return RES_BAD_ARG;  // no tracked paths are recorded
+  if (!e || !p || i >= e->paths.size()) return RES_BAD_ARG;
+  p->path__ = &e->paths[i];
+  return RES_OK;
 }
 
 res_T ssol_path_get_vertices_count(const struct ssol_path* p, size_t* n) {
   if (!p || !n || !p->path__) return RES_BAD_ARG;
-  return RES_BAD_ARG;
+  *n = static_cast<const ssol_estimator::tracked_path*>(p->path__)->n;
+  return RES_OK;
 }
 
-res_T ssol_path_get_vertex(const struct ssol_path* p, const size_t, struct ssol_path_vertex* v) {
-  if (!p || !v) return RES_BAD_ARG;
-  return RES_BAD_ARG;
+res_T ssol_path_get_vertex(const struct ssol_path* p, const size_t i, struct ssol_path_vertex* v) {
+  if (!p || !v || !p->path__) return RES_BAD_ARG;
+  const auto* tp = static_cast<const ssol_estimator::tracked_path*>(p->path__);
+  if (i >= tp->n) return RES_BAD_ARG;
+  *v = tp->v[i];
+  return RES_OK;
 }
 
 res_T ssol_path_get_type(const struct ssol_path* p, enum ssol_path_type* t) {
-  if (!p || !t) return RES_BAD_ARG;
-  return RES_BAD_ARG;
+  if (!p || !t || !p->path__) return RES_BAD_ARG;
+  *t = (enum ssol_path_type) static_cast<const ssol_estimator::tracked_path*>(p->path__)->type;
+  return RES_OK;
 }
 
 res_T ssol_mc_receiver_get_mc_shape(struct ssol_mc_receiver* rcv, const struct ssol_shape* shape,

@@ -160,6 +160,21 @@ class SolveStats(C.Structure):
                 ("iterations", C.c_uint)]
 
 
+class PathTracker(C.Structure):
+    """struct ssol_path_tracker: a negative length takes the default"""
+    _fields_ = [("sun_ray_length", C.c_double), ("infinite_ray_length", C.c_double)]
+
+
+class Path(C.Structure):
+    _fields_ = [("path__", C.c_void_p)]
+
+
+class PathVertex(C.Structure):
+    _fields_ = [("pos", C.c_double * 3), ("weight", C.c_double)]
+
+
+PATH_TYPES = ("missing", "shadow", "success", "error")  # enum ssol_path_type
+
 VP = C.c_void_p
 PVP = C.POINTER(C.c_void_p)
 
@@ -242,6 +257,11 @@ def _declare(L):
         "ssol_estimator_get_sampled_count": [VP, C.POINTER(C.c_size_t)],
         "ssol_estimator_get_rng_state": [VP, PVP],
         "ssol_estimator_get_solve_stats": [VP, C.POINTER(SolveStats)],
+        "ssol_estimator_get_tracked_paths_count": [VP, C.POINTER(C.c_size_t)],
+        "ssol_estimator_get_tracked_path": [VP, C.c_size_t, C.POINTER(Path)],
+        "ssol_path_get_vertices_count": [C.POINTER(Path), C.POINTER(C.c_size_t)],
+        "ssol_path_get_vertex": [C.POINTER(Path), C.c_size_t, C.POINTER(PathVertex)],
+        "ssol_path_get_type": [C.POINTER(Path), C.POINTER(C.c_int)],
         "ssol_mc_receiver_get_mc_shape": [C.POINTER(McReceiver), VP, C.POINTER(McShape)],
         "ssol_mc_shape_get_mc_primitive": [C.POINTER(McShape), C.c_uint, C.POINTER(McPrimitive)],
         "ssp_rng_create": [VP, C.c_int, PVP],
@@ -537,10 +557,16 @@ class Scene:
         return n.value
 
     def solve(self, n: int, seed: int = 0, counter: int = 0, max_failed: int = 0,
-              rng=None) -> "Estimator":
+              rng=None, tracker=None) -> "Estimator":
+        """tracker: None, or (sun_ray_length, infinite_ray_length) to record
+        every path (negative lengths take the default)"""
         r = rng if rng is not None else Rng(seed, counter)
         h = C.c_void_p()
-        res = lib().ssol_solve(self.scn, r.h if isinstance(r, Rng) else r, n, max_failed, None,
+        trk = None
+        if tracker is not None:
+            t = PathTracker(*tracker)
+            trk = C.cast(C.pointer(t), C.c_void_p)
+        res = lib().ssol_solve(self.scn, r.h if isinstance(r, Rng) else r, n, max_failed, trk,
                                C.byref(h))
         if res not in (0, 5) or not h.value:
             raise SsolError("ssol_solve", res)
@@ -645,6 +671,26 @@ class Estimator:
         h = C.c_void_p()
         call("ssol_estimator_get_rng_state", self.h, C.byref(h))
         return h
+
+    def tracked_paths(self):
+        """[(type name, vertices [n][4] = x, y, z, weight)] of a solve with a
+        path tracker, through ssol_estimator_get_tracked_path and the
+        ssol_path getters"""
+        n = C.c_size_t()
+        call("ssol_estimator_get_tracked_paths_count", self.h, C.byref(n))
+        out = []
+        path, nv, ty, v = Path(), C.c_size_t(), C.c_int(), PathVertex()
+        for i in range(n.value):
+            call("ssol_estimator_get_tracked_path", self.h, i, C.byref(path))
+            call("ssol_path_get_vertices_count", C.byref(path), C.byref(nv))
+            call("ssol_path_get_type", C.byref(path), C.byref(ty))
+            a = np.zeros((nv.value, 4))
+            for k in range(nv.value):
+                call("ssol_path_get_vertex", C.byref(path), k, C.byref(v))
+                a[k, :3] = v.pos[:]
+                a[k, 3] = v.weight
+            out.append((PATH_TYPES[ty.value], a))
+        return out
 
     @property
     def stats(self) -> SolveStats:
