@@ -120,11 +120,15 @@ struct ssolh_sshape {
 };
 
 /* Object = BLAS over the triangles of its shaded shapes */
+#define SSOLH_OBJECT_PLANAR 1u  /* mesh shapes only, all triangles in one
+                                   plane: a ray leaving the object can meet it
+                                   again only at t ~ 0, which the hit filter
+                                   rejects (t <= 1e-6) */
 struct ssolh_object {
   uint32_t sshape_first;    /* into sshapes */
   uint32_t sshape_count;
   uint32_t blas_root;       /* into nodes */
-  uint32_t pad__;
+  uint32_t flags;           /* SSOLH_OBJECT_* */
 };
 
 /* BVH node (BVH2, both child boxes stored in the parent). A child with

@@ -85,7 +85,7 @@ struct alignas(8) IssInfo {
   int32_t receiver;        // receiver slot of the instance or -1
   uint32_t receiver_mask;  // ssol_side_flag bits
   uint32_t shape_type;     // enum ssolh_shape_type
-  uint32_t pad;
+  uint32_t obj_flags;      // SSOLH_OBJECT_* of the instance's object
   int64_t prim_tally;      // first per-primitive slot or -1
 };
 
@@ -507,6 +507,13 @@ k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict_
   bool in_blas = false;
   uint32_t iss_first = 0;   // (instance, sshape) base of the BLAS being traversed
   const float ftmin = r.tmin;
+  // A ray that leaves a point of a planar object can meet that object again
+  // only at t ~ 0, which the filter rejects: its instance is not entered.
+  uint32_t skip_inst = NONE;
+  if (r.tmin == 0.f) {
+    const IssInfo e = S.iss[r.iss_from];
+    if (e.obj_flags & SSOLH_OBJECT_PLANAR) skip_inst = e.inst;
+  }
 
   // "while-while" traversal: the lanes of a wave first descend through inner
   // nodes together, then process their leaves together, which keeps the
@@ -557,7 +564,12 @@ k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict_
     const uint32_t first = ref & 0x0FFFFFFFu;
     if (!in_blas) {
       // TLAS leaf: one instance, enter its BLAS
-      const ssolh_instance& inst = S.insts[S.tlas_inst[first]];
+      const uint32_t ii = S.tlas_inst[first];
+      if (ii == skip_inst) {
+        alive = pop();
+        continue;
+      }
+      const ssolh_instance& inst = S.insts[ii];
       const d3 T = mk(inst.transform[9], inst.transform[10], inst.transform[11]);
       o = mul(inst.inv_linear, ow - T);
       d = mul(inst.inv_linear, dw);
@@ -808,7 +820,6 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
   const uint32_t slot = live ? slot_of(list, q) : 0;
 
   if (live) {
-    const RayRec r = rays[slot];
     const HitRec h = hits[slot];
     uint4 meta = P.meta[slot];
     uint4 cur = P.cur[slot];
@@ -817,7 +828,6 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
     double4 dc = P.dir_cos[slot];
     double4 pv = P.prev_init[slot];
     double4 oi = P.out_in0[slot];
-    double4 nt = P.nrm_t[slot];
     double4 pd = P.pend[slot];
     const uint64_t path = (uint64_t)meta.x | ((uint64_t)meta.y << 32);
     PathRng rng;
@@ -831,7 +841,12 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
     d3 pos = mk(pw.x, pw.y, pw.z);
     const double wl = pw.w;
     d3 dir = mk(dc.x, dc.y, dc.z);
-    d3 N = mk(nt.x, nt.y, nt.z);
+    d3 N = mk(0, 0, 0);
+    if (!(flags & PF_STAGE_BOUNCE)) {
+      // normal of the start point (later points take theirs from the hit)
+      const double4 nt = P.nrm_t[slot];
+      N = mk(nt.x, nt.y, nt.z);
+    }
     double hit_t = 0;
     d3 rorg = pos, rdir = dir;  // origin/direction of the ray of the point
     bool head = false;
@@ -857,7 +872,7 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
         last = true;
         // escape point (src/ssol_solver.c:1012-1019)
         if (K.vtx && K.inf_len > 0)
-          track_vertex(K, slot, path, pos + mk(r.dx, r.dy, r.dz) * K.inf_len, oi.x);
+          track_vertex(K, slot, path, pos + dir * K.inf_len, oi.x);
         if (out_med != S.air) fin.kind = FIN_FAILED;  // ray lost outside air
       }
       if (fin.kind == FIN_NONE && (last || !hit_virtual)) {
@@ -883,8 +898,15 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
       }
       if (fin.kind == FIN_NONE) {
         // point_update_from_hit (src/ssol_solver.c:360-411)
-        rorg = mk(r.ox, r.oy, r.oz);
-        rdir = mk(r.dx, r.dy, r.dz);
+        // the ray left the current point along dir, except that a ray past
+        // a virtual point keeps the origin of the ray that reached it
+        if (hit_virtual) {
+          const RayRec& rr = rays[slot];
+          rorg = mk(rr.ox, rr.oy, rr.oz);
+        } else {
+          rorg = pos;
+        }
+        rdir = dir;
         iss = h.iss;
         prim = h.prim;
         N = normalize(mk(h.nx, h.ny, h.nz));
@@ -1043,7 +1065,6 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
       P.dir_cos[slot] = make_double4(dir.x, dir.y, dir.z, dc.w);
       P.prev_init[slot] = pv;
       P.out_in0[slot] = oi;
-      P.nrm_t[slot] = make_double4(N.x, N.y, N.z, hit_t);
       P.pend[slot] = pd;
       P.meta[slot] = make_uint4(meta.x, meta.y, rng.block, flags);
       P.cur[slot] = make_uint4(iss, prim, side, cur.w);
@@ -1398,6 +1419,7 @@ int build_iss(const ssolh_scene* s, std::vector<IssInfo>& out) {
       d.receiver = in.receiver;
       d.receiver_mask = in.receiver_mask;
       d.shape_type = (uint32_t)s->shapes[sh.shape].type;
+      d.obj_flags = o.flags;
       d.prim_tally = s->inst_sshapes[g].prim_tally;
     }
   }

@@ -197,6 +197,12 @@ k_trace32(DScene S, const ShapeQ* __restrict__ SQ, const uint32_t* __restrict__ 
   }
   const FRay fw = fr;
   FShear sh = make_shear32(dw);
+  // a ray leaving a planar object does not enter that object again
+  uint32_t skip_inst = NONE;
+  if (r.tmin == 0.f) {
+    const IssInfo e = S.iss[r.iss_from];
+    if (e.obj_flags & SSOLH_OBJECT_PLANAR) skip_inst = e.inst;
+  }
   bool in_blas = false;
   uint32_t iss_first = 0;
   const float ftmin = r.tmin;
@@ -243,6 +249,10 @@ k_trace32(DScene S, const ShapeQ* __restrict__ SQ, const uint32_t* __restrict__ 
     const uint32_t first = ref & 0x0FFFFFFFu;
     if (!in_blas) {
       const uint32_t ii = S.tlas_inst[first];
+      if (ii == skip_inst) {
+        alive = pop();
+        continue;
+      }
       d3 od, dd;
       to_instance(S, r.base, off, dw, ii, od, dd);
       o = tof(od);
@@ -472,7 +482,6 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
     float4 dw = P.dir_wl[slot];
     float4 pv = P.prev_init[slot];
     float4 oi = P.out_in0[slot];
-    float4 nt = P.nrm_t[slot];
     float4 pc = P.pend_cos[slot];
     const uint64_t path = (uint64_t)meta.x | ((uint64_t)meta.y << 32);
     PathRng32 rng;
@@ -487,7 +496,11 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
     uint32_t base = __float_as_uint(ps.w);
     const float wl = dw.w;
     f3 dir = mkf(dw.x, dw.y, dw.z);
-    f3 N = mkf(nt.x, nt.y, nt.z);
+    f3 N = mkf(0, 0, 0);
+    if (!(flags & PF_STAGE_BOUNCE)) {
+      const float4 nt = P.nrm_t[slot];
+      N = mkf(nt.x, nt.y, nt.z);
+    }
     float hit_t = 0;
     bool head = false;
     uint32_t iss = cur.x, prim = cur.y, side = cur.z;
@@ -661,7 +674,6 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
       P.dir_wl[slot] = make_float4(dir.x, dir.y, dir.z, wl);
       P.prev_init[slot] = pv;
       P.out_in0[slot] = oi;
-      P.nrm_t[slot] = make_float4(N.x, N.y, N.z, hit_t);
       P.pend_cos[slot] = pc;
       P.meta[slot] = make_uint4(meta.x, meta.y, rng.block, flags);
       P.cur[slot] = make_uint4(iss, prim, side, cur.w);

@@ -226,6 +226,50 @@ void inflate(aabb& b, float rel, float abs_) {
   for (int k = 0; k < 3; ++k) { b.lo[k] -= e; b.hi[k] += e; }
 }
 
+// Whether every shape of `o` is a mesh and all its vertices lie in one plane
+// (to 1e-12 of the object's size, i.e. exactly for float data such as a
+// facet grid in z = 0).
+bool object_is_planar(const ssol_object* o) {
+  double p0[3] = {0, 0, 0}, n[3] = {0, 0, 0}, best = 0, lo[3], hi[3];
+  bool first = true;
+  for (int k = 0; k < 3; ++k) { lo[k] = DBL_MAX; hi[k] = -DBL_MAX; }
+  for (const auto& ss : o->sshapes) {
+    if (ss.shape->kind != shape_kind::mesh) return false;
+    const std::vector<float>& P = ss.shape->pos;
+    const std::vector<unsigned>& I = ss.shape->ids;
+    for (size_t v = 0; v + 2 < P.size(); v += 3) {
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], (double)P[v + k]);
+        hi[k] = std::max(hi[k], (double)P[v + k]);
+      }
+      if (first) { for (int k = 0; k < 3; ++k) p0[k] = P[v + k]; first = false; }
+    }
+    for (size_t t = 0; t + 2 < I.size(); t += 3) {  // normal of the largest triangle
+      const float* a = &P[3 * I[t]];
+      const float* b = &P[3 * I[t + 1]];
+      const float* c = &P[3 * I[t + 2]];
+      const double e1[3] = {(double)b[0] - a[0], (double)b[1] - a[1], (double)b[2] - a[2]};
+      const double e2[3] = {(double)c[0] - a[0], (double)c[1] - a[1], (double)c[2] - a[2]};
+      const double m[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                           e1[0] * e2[1] - e1[1] * e2[0]};
+      const double a2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+      if (a2 > best) { best = a2; for (int k = 0; k < 3; ++k) n[k] = m[k]; }
+    }
+  }
+  if (first || !(best > 0)) return false;
+  const double len = std::sqrt(best);
+  double ext = 0;
+  for (int k = 0; k < 3; ++k) { n[k] /= len; ext = std::max(ext, hi[k] - lo[k]); }
+  for (const auto& ss : o->sshapes) {
+    const std::vector<float>& P = ss.shape->pos;
+    for (size_t v = 0; v + 2 < P.size(); v += 3) {
+      const double dist = n[0] * (P[v] - p0[0]) + n[1] * (P[v + 1] - p0[1]) + n[2] * (P[v + 2] - p0[2]);
+      if (std::fabs(dist) > 1e-12 * ext) return false;
+    }
+  }
+  return true;
+}
+
 res_T object_id(Flat& f, const ssol_object* o, uint32_t* out) {
   auto it = f.obj_index.find(o);
   if (it != f.obj_index.end()) { *out = it->second; return RES_OK; }
@@ -259,6 +303,7 @@ res_T object_id(Flat& f, const ssol_object* o, uint32_t* out) {
       boxes.push_back(b);
     }
   }
+  if (object_is_planar(o)) d.flags |= SSOLH_OBJECT_PLANAR;
   std::vector<uint32_t> order;
   const size_t node0 = f.nodes.size();
   unsigned depth = 0;
