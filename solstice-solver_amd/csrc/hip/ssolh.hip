@@ -798,11 +798,14 @@ __device__ void spill(double* base, uint64_t key, const double* w, unsigned long
   }
 }
 
+// ADJ: the adjoint solve's variant; the plain one carries no score code
+template <bool ADJ>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
 k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __restrict__ list,
         const unsigned* __restrict__ count_ptr, RayRec* __restrict__ rays,
         const HitRec* __restrict__ hits, uint8_t* __restrict__ emitted,
-        uint32_t* __restrict__ block_emits, Counters* C, AdjParams A, TrackParams K) {
+        uint32_t* __restrict__ block_emits, Counters* C, AdjParams A_, TrackParams K) {
+  const AdjParams A = ADJ ? A_ : AdjParams{};
   const uint32_t count = *count_ptr;
   if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: block_count syncs
   const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
@@ -1788,9 +1791,14 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
                                                  (RayRec32*)c->rays.p, (const HitRec32*)c->hits.p,
                                                  emitted, block_emits, ctr);
     else
-      k_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
-                                               (RayRec*)c->rays.p, (const HitRec*)c->hits.p,
-                                               emitted, block_emits, ctr, A, K);
+      if (A.grad)
+        k_shade<true><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
+                                                      (RayRec*)c->rays.p, (const HitRec*)c->hits.p,
+                                                      emitted, block_emits, ctr, A, K);
+      else
+        k_shade<false><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
+                                                       (RayRec*)c->rays.p, (const HitRec*)c->hits.p,
+                                                       emitted, block_emits, ctr, A, K);
     HIP_TRY(hipGetLastError());
     if (!graph_iters) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
     k_scan<<<1, SCAN_THREADS, 0, c->stream>>>(block_emits, count, block_offsets, cnt + (i ^ 1),
