@@ -184,10 +184,33 @@ def adjoint_timing(n=20_000_000):
     t0 = time.perf_counter()
     g = lib.solve_adjoint(n, spec.instances[:-1], axes, d_absorbed=1.0, seed=2)
     t_adj = time.perf_counter() - t0
+
+    # one aiming step (new aim angles for every heliostat, forward and
+    # adjoint): moving the instances of the resident scene, which rebuilds
+    # only instance frames and the TLAS, against building the scene anew
+    aim2 = aim + 1e-4
+    spec2 = S.config2(0, n_helio=len(aim), radius=900.0, aim_perturb=aim2, slope_errors=slope)
+    axes2 = S.aim_axes(0, len(aim), 900.0, None, aim2)
+    steps = {}
+    for kind in ("move", "rebuild"):
+        t0 = time.perf_counter()
+        if kind == "move":
+            for a, b in zip(spec.instances[:-1], spec2.instances[:-1]):
+                lib.set_transform(a, b.transform)
+            sc, helios = lib, spec.instances[:-1]
+        else:
+            sc, helios = L.Scene(spec2), spec2.instances[:-1]
+        sc.solve(n, seed=3)
+        sc.solve_adjoint(n, helios, axes2, d_absorbed=1.0, seed=3)
+        steps[kind] = time.perf_counter() - t0
+        if kind == "rebuild":
+            sc.close()
     lib.close()
     out = {"heliostats": len(spec.instances) - 1, "paths": n, "forward_s": round(t_fwd, 4),
            "adjoint_s": round(t_adj, 4), "adjoint_Mpaths_s": round(n / t_adj / 1e6, 2),
-           "grad_norm": float(np.linalg.norm(g))}
+           "grad_norm": float(np.linalg.norm(g)),
+           "aim_step_s_moving_instances": round(steps["move"], 4),
+           "aim_step_s_rebuilding_scene": round(steps["rebuild"], 4)}
     print(f"[bench] adjoint: {out}", file=sys.stderr, flush=True)
     return out
 

@@ -53,6 +53,7 @@ struct Flat {
   std::vector<ssolh_inst_sshape> isshapes;
   std::vector<ssolh_sample_tri> stris;
   std::vector<ssolh_alias> alias;
+  size_t blas_nodes = 0;      // nodes of the BLASes; the TLAS follows them
   ssolh_scene scn{};
 };
 
@@ -432,8 +433,11 @@ void push_medium_sig(std::vector<uint64_t>& sig, const ssol_medium& m) {
 
 // Everything flatten() reads, so that an unchanged scene is not flattened
 // and uploaded again
-std::vector<uint64_t> scene_signature(const ssol_scene* scn) {
+// Instance transforms go to *xf, the rest to the returned signature: scenes
+// that differ only in *xf share their objects, sampling and tally layout.
+std::vector<uint64_t> scene_signature(const ssol_scene* scn, std::vector<uint64_t>* xf) {
   std::vector<uint64_t> sig;
+  xf->clear();
   auto P = [&](const void* p) { sig.push_back((uint64_t)(uintptr_t)p); };
   P(scn);
   P(scn->sun);
@@ -451,7 +455,7 @@ std::vector<uint64_t> scene_signature(const ssol_scene* scn) {
     for (int k = 0; k < 12; ++k) {
       uint64_t b;
       memcpy(&b, &in->transform[k], 8);
-      sig.push_back(b);
+      xf->push_back(b);
     }
     sig.push_back((uint64_t)in->receiver_mask);
     sig.push_back((uint64_t)in->per_primitive);
@@ -512,6 +516,85 @@ res_T flat_sun(Flat& f, const ssol_sun* sun, bool add_spectrum, ssolh_sun& su) {
   return RES_OK;
 }
 
+// World transform of an instance and, per shaded shape, the world <->
+// quadric frames of punched shapes (src/ssol_shape.c:1461-1466)
+void instance_frames(const ssol_instance* in, ssolh_instance& d, ssolh_inst_sshape* iss) {
+  memcpy(d.transform, in->transform, sizeof(d.transform));
+  mat_inv(in->transform, d.inv_linear);
+  for (size_t s = 0; s < in->object->sshapes.size(); ++s) {
+    const ssol_shape* sh = in->object->sshapes[s].shape;
+    ssolh_inst_sshape& is = iss[s];
+    if (sh->kind == shape_kind::punched) {
+      mat_mul(in->transform, sh->qtransform, is.R);
+      for (int k = 0; k < 3; ++k)
+        is.T[k] = in->transform[k] * sh->qtransform[9] + in->transform[3 + k] * sh->qtransform[10] +
+                  in->transform[6 + k] * sh->qtransform[11] + in->transform[9 + k];
+      mat_inv(is.R, is.R_inv);
+    } else {
+      memset(is.R, 0, sizeof(is.R));
+      memset(is.T, 0, sizeof(is.T));
+      memset(is.R_inv, 0, sizeof(is.R_inv));
+      is.R[0] = is.R[4] = is.R[8] = 1;
+      is.R_inv[0] = is.R_inv[4] = is.R_inv[8] = 1;
+    }
+  }
+}
+
+// TLAS over the world boxes of the instances, after the BLAS nodes
+res_T build_tlas(ssol_device* dev, Flat& f) {
+  f.nodes.resize(f.blas_nodes);
+  f.tlas_inst.clear();
+  std::vector<aabb> boxes(f.instances.size());
+  unsigned max_blas = 0;
+  for (size_t i = 0; i < f.instances.size(); ++i) {
+    const ssolh_instance& in = f.instances[i];
+    const aabb& ob = f.obj_box[in.object];
+    max_blas = std::max(max_blas, f.obj_depth[in.object]);
+    aabb& b = boxes[i];
+    for (int k = 0; k < 3; ++k) { b.lo[k] = FLT_MAX; b.hi[k] = -FLT_MAX; }
+    for (int c = 0; c < 8; ++c) {
+      const double p[3] = {(c & 1) ? ob.hi[0] : ob.lo[0], (c & 2) ? ob.hi[1] : ob.lo[1],
+                           (c & 4) ? ob.hi[2] : ob.lo[2]};
+      for (int k = 0; k < 3; ++k) {
+        const double w = in.transform[k] * p[0] + in.transform[3 + k] * p[1] +
+                         in.transform[6 + k] * p[2] + in.transform[9 + k];
+        b.lo[k] = std::min(b.lo[k], (float)w);
+        b.hi[k] = std::max(b.hi[k], (float)w);
+      }
+    }
+    if (ob.lo[0] > ob.hi[0]) {  // empty object: box that no ray reaches
+      for (int k = 0; k < 3; ++k) { b.lo[k] = 0; b.hi[k] = 0; }
+    }
+    inflate(b, 2e-6f, 1e-6f);
+  }
+  unsigned tdepth = 0;
+  f.scn.tlas_root = bvh_build(boxes, 1, f.nodes, f.tlas_inst, &tdepth);
+  if (tdepth + max_blas + 2 > (unsigned)GPU_STACK) {
+    log_error(dev, "ssol_solve: BVH too deep for the GPU traversal stack (%u + %u levels).\n",
+              tdepth, max_blas);
+    return RES_MEM_ERR;
+  }
+  f.scn.stack_depth = tdepth + max_blas + 2;
+  f.scn.nnodes = (uint32_t)f.nodes.size();
+  f.scn.nodes = f.nodes.data();
+  f.scn.tlas_instances = f.tlas_inst.data();
+  log_warning(dev, "ssol_solve: %zu instances, %zu triangles, TLAS depth %u, BLAS depth %u.\n",
+              f.instances.size(), f.tris.size(), tdepth, max_blas);
+  return RES_OK;
+}
+
+// Instances moved, nothing else changed: new instance frames and a new TLAS
+// over the same objects, sampling tables and tally layout (the sampled
+// areas are object-space areas, which rigid motions keep).
+res_T refresh_transforms(ssol_scene* scn, Flat& f) {
+  if (f.instances.size() != scn->instances.size()) return RES_BAD_ARG;
+  for (size_t i = 0; i < scn->instances.size(); ++i) {
+    ssolh_instance& d = f.instances[i];
+    instance_frames(scn->instances[i], d, f.isshapes.data() + d.isshape_first);
+  }
+  return build_tlas(scn->dev, f);
+}
+
 res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
   ssol_device* dev = scn->dev;
   ssol_sun* sun = scn->sun;
@@ -546,8 +629,6 @@ res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
       f.sshapes[obj.sshape_first + s].mtl_back = mb;
     }
     ssolh_instance d{};
-    memcpy(d.transform, in->transform, sizeof(d.transform));
-    mat_inv(in->transform, d.inv_linear);
     d.object = oid;
     d.receiver_mask = (uint32_t)in->receiver_mask;
     d.receiver = in->receiver_mask ? (int32_t)L.nrcv++ : -1;
@@ -557,19 +638,11 @@ res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
     L.rcv_slot.push_back(d.receiver);
     L.samp_slot.push_back(d.sampled);
     if (in->receiver_mask) has_receiver = true;
+    std::vector<ssolh_inst_sshape> frames(in->object->sshapes.size());
+    instance_frames(in, d, frames.data());
     for (size_t s = 0; s < in->object->sshapes.size(); ++s) {
       const ssol_shape* sh = in->object->sshapes[s].shape;
-      ssolh_inst_sshape is{};
-      if (sh->kind == shape_kind::punched) {
-        mat_mul(in->transform, sh->qtransform, is.R);
-        for (int k = 0; k < 3; ++k)
-          is.T[k] = in->transform[k] * sh->qtransform[9] + in->transform[3 + k] * sh->qtransform[10] +
-                    in->transform[6 + k] * sh->qtransform[11] + in->transform[9 + k];
-        mat_inv(is.R, is.R_inv);
-      } else {
-        is.R[0] = is.R[4] = is.R[8] = 1;
-        is.R_inv[0] = is.R_inv[4] = is.R_inv[8] = 1;
-      }
+      ssolh_inst_sshape is = frames[s];
       is.prim_tally = -1;
       if (in->receiver_mask && in->per_primitive) {
         is.prim_tally = (int64_t)L.nprim;
@@ -612,40 +685,8 @@ res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
   if (!has_receiver) log_warning(dev, "No receiver is defined.\n");
   build_alias(weights, f.alias);
 
-  // TLAS over the world boxes of the instances
-  std::vector<aabb> boxes(f.instances.size());
-  unsigned max_blas = 0;
-  for (size_t i = 0; i < f.instances.size(); ++i) {
-    const ssolh_instance& in = f.instances[i];
-    const aabb& ob = f.obj_box[in.object];
-    max_blas = std::max(max_blas, f.obj_depth[in.object]);
-    aabb& b = boxes[i];
-    for (int k = 0; k < 3; ++k) { b.lo[k] = FLT_MAX; b.hi[k] = -FLT_MAX; }
-    for (int c = 0; c < 8; ++c) {
-      const double p[3] = {(c & 1) ? ob.hi[0] : ob.lo[0], (c & 2) ? ob.hi[1] : ob.lo[1],
-                           (c & 4) ? ob.hi[2] : ob.lo[2]};
-      for (int k = 0; k < 3; ++k) {
-        const double w = in.transform[k] * p[0] + in.transform[3 + k] * p[1] +
-                         in.transform[6 + k] * p[2] + in.transform[9 + k];
-        b.lo[k] = std::min(b.lo[k], (float)w);
-        b.hi[k] = std::max(b.hi[k], (float)w);
-      }
-    }
-    if (ob.lo[0] > ob.hi[0]) {  // empty object: box that no ray reaches
-      for (int k = 0; k < 3; ++k) { b.lo[k] = 0; b.hi[k] = 0; }
-    }
-    inflate(b, 2e-6f, 1e-6f);
-  }
-  unsigned tdepth = 0;
-  f.scn.tlas_root = bvh_build(boxes, 1, f.nodes, f.tlas_inst, &tdepth);
-  if (tdepth + max_blas + 2 > (unsigned)GPU_STACK) {
-    log_error(dev, "ssol_solve: BVH too deep for the GPU traversal stack (%u + %u levels).\n",
-              tdepth, max_blas);
-    return RES_MEM_ERR;
-  }
-  f.scn.stack_depth = tdepth + max_blas + 2;
-  log_warning(dev, "ssol_solve: %zu instances, %zu triangles, TLAS depth %u, BLAS depth %u.\n",
-              f.instances.size(), f.tris.size(), tdepth, max_blas);
+  f.blas_nodes = f.nodes.size();
+  if (res_T r = build_tlas(dev, f)) return r;
 
   // sun (src/ssol_sun.c:214-279)
   {
@@ -671,13 +712,10 @@ res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
   s.sshapes = f.sshapes.data();
   s.nobjects = (uint32_t)f.objects.size();
   s.objects = f.objects.data();
-  s.nnodes = (uint32_t)f.nodes.size();
-  s.nodes = f.nodes.data();
   s.ntriangles = (uint32_t)f.tris.size();
   s.triangles = f.tris.data();
   s.ninstances = (uint32_t)f.instances.size();
   s.instances = f.instances.data();
-  s.tlas_instances = f.tlas_inst.data();
   s.ninst_sshapes = (uint32_t)f.isshapes.size();
   s.inst_sshapes = f.isshapes.data();
   s.nsample_tris = (uint32_t)f.stris.size();
@@ -717,7 +755,7 @@ void fill_receiver(const double* base, double N, ssol_mc_receiver* r) {
 }  // namespace
 
 struct solve_cache {
-  std::vector<uint64_t> sig;
+  std::vector<uint64_t> sig, xf;
   Flat flat;
   Layout layout;
 };
@@ -736,12 +774,21 @@ namespace {
 // generation stamp), creates the GPU context at the first solve.
 res_T prepare_scene(ssol_scene* scn) {
   ssol_device* dev = scn->dev;
-  std::vector<uint64_t> sig = scene_signature(scn);
-  const bool cached = dev->cache && dev->cache->sig == sig;
-  if (!cached) {
+  std::vector<uint64_t> xf;
+  std::vector<uint64_t> sig = scene_signature(scn, &xf);
+  const bool same = dev->cache && dev->cache->sig == sig;
+  const bool cached = same && dev->cache->xf == xf;
+  if (!same) {
     solve_cache_free(dev);
     dev->cache = new solve_cache();
     res_T r = flatten(scn, dev->cache->flat, dev->cache->layout);
+    if (r != RES_OK) {
+      solve_cache_free(dev);
+      return r;
+    }
+  } else if (!cached) {
+    // only instances moved: keep the objects, rebuild frames and the TLAS
+    res_T r = refresh_transforms(scn, dev->cache->flat);
     if (r != RES_OK) {
       solve_cache_free(dev);
       return r;
@@ -765,6 +812,7 @@ res_T prepare_scene(ssol_scene* scn) {
       return e == SSOLH_ERR_ARG ? RES_BAD_ARG : RES_MEM_ERR;
     }
     dev->cache->sig.swap(sig);
+    dev->cache->xf.swap(xf);
   }
   return RES_OK;
 }

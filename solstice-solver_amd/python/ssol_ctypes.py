@@ -405,6 +405,12 @@ class Scene:
             self._handles.remove(("ssol_sun_ref_put", h))
         return res, list(got)
 
+    def set_transform(self, inst, transform):
+        """ssol_instance_set_transform on the instance built from spec `inst`
+        (the next solve moves it without rebuilding the objects)"""
+        call("ssol_instance_set_transform", self.instances[id(inst)],
+             (C.c_double * 12)(*transform))
+
     def solve_adjoint(self, n: int, heliostats, axes, d_absorbed: float = 0.0,
                       map_receiver=None, map_shape=None, d_map=None, seed: int = 0,
                       counter: int = 0) -> np.ndarray:
