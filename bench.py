@@ -126,40 +126,42 @@ def other_forms(spec, B, ttt_batch, args):
             entry["time_to_0.1pct_rel_se_s"] = round(t, 4)
         out[name] = entry
         print(f"[bench] {name}: {entry}", file=sys.stderr, flush=True)
+        del est
         lib.close()
     return out
 
 
-def sweep(n_suns=512, log_min=5.0, log_max=7.0, seq_suns=64):
+def sweep(n_suns=512, log_min=5.0, log_max=7.0, seq_suns=64, graph=0):
     """The 100-paraboloid field under n_suns sun positions with seeded
-    log-uniform realisation counts: one batched solve (graph-captured) and,
-    for comparison, the first seq_suns suns as separate ssol_solve calls
-    through the public API (a sun change re-flattens the scene, as any
+    log-uniform realisation counts: one batched solve and, for comparison,
+    the first seq_suns suns as separate ssol_solve calls on the same scene
+    through the public API (each sun change re-flattens the scene, as any
     caller's per-sun solve would)."""
+    import ctypes as C
     import ssol_ctypes as L
     import ssol_scene as S
     spec, dirs, counts = S.config4(n_suns=n_suns, log_min=log_min, log_max=log_max)
-    lib = L.Scene(spec, graph_iterations=16)
+    lib = L.Scene(spec, graph_iterations=graph)
     lib.solve_sun_batch(dirs[:4], [10000] * 4, seed=1)  # warm-up
     t0 = time.perf_counter()
     res, got = lib.solve_sun_batch(dirs, counts, seed=2)
     t_batch = time.perf_counter() - t0
     lib.close()
     n_total = int(sum(int(c) for c in counts))
-    # sequential: one scene per sun position (the reference's workflow)
+    # sequential: one solve per sun position on one scene
+    sl = L.Scene(spec)
+    sl.solve(10000, seed=1)  # warm-up
     t0 = time.perf_counter()
     n_seq = 0
     for j in range(seq_suns):
-        one = S.Scene(spec.instances, S.Sun(spec.sun.kind, dirs[j], spec.sun.dni,
-                                            spec.sun.spectrum, spec.sun.param),
-                      spec.atmosphere, name="one")
-        sl = L.Scene(one)
+        L.call("ssol_sun_set_direction", sl.sun, (C.c_double * 3)(*dirs[j]))
         sl.solve(int(counts[j]), seed=3)
-        sl.close()
         n_seq += int(counts[j])
     t_seq = time.perf_counter() - t0
+    sl.close()
     out = {"suns": n_suns, "paths": n_total, "batched_s": round(t_batch, 4),
            "batched_Mpaths_s": round(n_total / t_batch / 1e6, 2),
+           "graph_iterations": graph,
            "sequential_sample_suns": seq_suns, "sequential_sample_paths": n_seq,
            "sequential_sample_s": round(t_seq, 4),
            "sequential_Mpaths_s": round(n_seq / t_seq / 1e6, 2),
@@ -345,6 +347,11 @@ def main(argv=None):
         out["paths_to_0.1pct_rel_se"] = acc[2]
         print(f"[bench] 0.1% rel. SE on absorbed after {acc[2]} paths in {dt:.3f} s",
               file=sys.stderr, flush=True)
+
+    # the default wavefront holds every realisation of a step in device
+    # memory: free it before the other forms allocate theirs
+    del est
+    lib.close()
 
     if not args.no_forms and args.config == "c3":
         out["forms"] = other_forms(spec, B, min(B, 4_000_000), args)

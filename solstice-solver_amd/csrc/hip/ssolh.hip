@@ -1585,7 +1585,27 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   if (trk) graph_iters = 0;
   else c->trk_paths = 0;  // the recorded paths belong to the last solve only
   if (npend > MAX_PEND) { set_err("ssolh_solve: at most %d pending entries", MAX_PEND); return SSOLH_ERR_LIMIT; }
-  uint64_t slots64 = path_slots ? path_slots : (1u << 21);
+  // Wavefront width. By default every realisation is in flight at once when
+  // 2/5 of the device memory holds their state: the realisations then start
+  // together and each iteration traces one generation of rays (shadow rays,
+  // first reflections, ...), which keeps the waves coherent; larger solves
+  // run as many as fit, each slot taking its next realisation in place.
+  uint64_t slots64 = path_slots;
+  if (!slots64) {
+    const size_t real = fp32 ? sizeof(float) : sizeof(double);
+    const size_t per_slot = 6 * 4 * real + 3 * sizeof(uint4) +
+                            (size_t)npend * 2 * (sizeof(uint32_t) + 10 * real) +
+                            (fp32 ? sizeof(RayRec32) + sizeof(HitRec32) : sizeof(RayRec) + sizeof(HitRec)) +
+                            2 * sizeof(uint32_t) + 1 + (adj ? sizeof(double4) + sizeof(uint32_t) : 0) +
+                            (trk ? sizeof(uint32_t) : 0);
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t fit = (uint64_t)(total_b / 5 * 2) / per_slot;
+    slots64 = std::min<uint64_t>(std::max<uint64_t>(fit, 1u << 20), 1ull << 27);
+    // a captured iteration launches the grid of the whole width even when
+    // its list is short: graph mode keeps the width moderate
+    if (graph_iters) slots64 = std::min<uint64_t>(slots64, 1u << 22);
+  }
   if (slots64 > N) slots64 = N;
   slots64 = (slots64 + BLOCK - 1) / BLOCK * BLOCK;
   if (slots64 > (1ull << 28)) { set_err("ssolh_solve: at most 2^28 path slots"); return SSOLH_ERR_LIMIT; }

@@ -1340,13 +1340,14 @@ res_T ssol_device_autotune(struct ssol_device* dev, struct ssol_scene* scn, cons
   // candidates: path slots (the wavefront width) x iterations per graph
   // launch (0: host-driven iterations); each timed on a probe solve after
   // one warm-up, the fastest kept on the device
-  static const uint32_t slot_choices[] = {1u << 19, 1u << 20, 1u << 21, 1u << 22};
+  // (0: the memory-sized default width)
+  static const uint32_t slot_choices[] = {0, 1u << 20, 1u << 22, 1u << 24};
   static const uint32_t graph_choices[] = {0, 8, 32};
   double best = 1e300;
   uint32_t best_slots = 0, best_graph = 0;
   unsigned ntried = 0;
   for (uint32_t sl : slot_choices) {
-    if (sl > 4 * probe && sl != slot_choices[0]) continue;
+    if (sl >= probe) continue;  // the default already holds the whole probe
     for (uint32_t gi : graph_choices) {
       ssolh_solve_args a{};
       a.realisations = probe;

@@ -87,7 +87,8 @@ def test_autotune_keeps_a_candidate_and_stays_correct(gpu):
     spec = S.config1()
     sc = L.Scene(spec)
     t = sc.autotune(400_000)
-    assert t.candidates >= 3 and t.path_slots >= (1 << 19) and t.paths_per_second > 0
+    assert t.candidates >= 3 and t.path_slots in (0, 1 << 20, 1 << 22, 1 << 24)
+    assert t.paths_per_second > 0
     e = sc.solve(1_000_000, seed=2)
     a = e.global_()["absorbed_by_receivers"]
     assert abs(a[0] - S.CONFIG1_CLOSED_FORM["absorbed_by_receivers"]) <= 4 * a[2] + 1e-6 * 22500
