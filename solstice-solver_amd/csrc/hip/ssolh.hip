@@ -20,9 +20,9 @@
 //               tallies); finished realisations flush their pending tallies
 //               with wave-aggregated FP64 atomics and the slot is refilled
 //               with a new realisation in place;
-//   k_scan / k_scatter  order-preserving compaction of the slots that wrote
-//               a ray into the next work list (per-block counts, one-block
-//               scan of the counts, scatter).
+//   k_scan_chunks / k_scan_top / k_scatter  order-preserving compaction of
+//               the slots that wrote a ray into the next work list (per-block
+//               counts, two-level scan of the counts, scatter).
 //
 // The host loop runs trace / shade / scan / scatter until the list is empty.
 #include <hip/hip_runtime.h>
@@ -47,7 +47,7 @@ namespace {
 constexpr int BLOCK = 128;           // threads per block (2 waves)
 constexpr int WAVES = BLOCK / 64;
 constexpr int MAX_STACK = 48;        // traversal stack entries per thread (LDS), largest kernel
-constexpr int SCAN_THREADS = 1024;   // the one block of k_scan
+constexpr int SCAN_THREADS = 1024;   // threads of a scan block
 constexpr uint32_t SENTINEL = 0xFFFFFFFFu;
 constexpr uint32_t LEAF = 0x80000000u;
 constexpr uint32_t NONE = 0xFFFFFFFFu;
@@ -165,7 +165,7 @@ struct PathState {
 struct Counters {
   unsigned long long failed;
   unsigned long long spills;
-  unsigned long long segments;      // traced rays (added by k_scan, one thread)
+  unsigned long long segments;      // traced rays (added by k_scan_top, one thread)
   unsigned int aborted;
   unsigned int pad;
 };
@@ -798,8 +798,10 @@ __device__ void spill(double* base, uint64_t key, const double* w, unsigned long
   }
 }
 
-// ADJ: the adjoint solve's variant; the plain one carries no score code
-template <bool ADJ>
+// ADJ: the adjoint solve's variant; the plain one carries no score code.
+// REFILL: slots take further realisations (fewer slots than realisations);
+// without it the kernel carries no generation code.
+template <bool ADJ, bool REFILL>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
 k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __restrict__ list,
         const unsigned* __restrict__ count_ptr, RayRec* __restrict__ rays,
@@ -1153,7 +1155,7 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
   const uint64_t id = want ? (((uint64_t)P.meta[slot].y << 32) | P.meta[slot].x) + prm.slots : 0;
   int samp_slot = 0;
   bool gen = false;
-  if (want && id < prm.N) {
+  if (REFILL && want && id < prm.N) {
     generate(S, P, prm, slot, id, e, samp_slot, A, K);
     gen = true;
   } else if (live && fin.kind != FIN_NONE) {
@@ -1170,32 +1172,66 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
 // -------------------------------------------------------------------------
 // Order-preserving compaction of the work list
 // -------------------------------------------------------------------------
-// Exclusive scan of the per-block emit counts of k_shade, by one block;
-// the grand total is the next list's length.
-__global__ void __launch_bounds__(SCAN_THREADS)
-k_scan(const uint32_t* __restrict__ counts, const unsigned* __restrict__ count_ptr,
-       uint32_t* __restrict__ offsets, unsigned int* total, unsigned long long* segments) {
-  __shared__ uint32_t part[SCAN_THREADS];
+// Exclusive scan of the per-block emit counts of k_shade in two levels:
+// k_scan_chunks scans each chunk of SCAN_CHUNK counts (offsets within the
+// chunk) and records the chunk's total; k_scan_top scans the chunk totals in
+// place into chunk offsets and writes the grand total, the next list's
+// length. Entry b of the scan is offsets[b] + chunk[b / SCAN_CHUNK].
+constexpr int SCAN_PER = 4;
+constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_PER;
+static_assert((1u << 28) / BLOCK / SCAN_CHUNK <= SCAN_THREADS, "one k_scan_top block scans every chunk");
+
+__device__ void block_inclusive_scan(uint32_t* part) {
   const uint32_t tid = threadIdx.x;
-  const uint32_t nblocks = (*count_ptr + BLOCK - 1) / BLOCK;
-  if (tid == 0) *segments += *count_ptr;
-  const uint32_t per = (nblocks + SCAN_THREADS - 1) / SCAN_THREADS;
-  const uint32_t b0 = min(tid * per, nblocks), b1 = min(b0 + per, nblocks);
-  uint32_t s = 0;
-  for (uint32_t b = b0; b < b1; ++b) s += counts[b];
-  part[tid] = s;
-  __syncthreads();
   for (uint32_t off = 1; off < SCAN_THREADS; off <<= 1) {
     const uint32_t v = tid >= off ? part[tid - off] : 0u;
     __syncthreads();
     part[tid] += v;
     __syncthreads();
   }
-  uint32_t run = tid ? part[tid - 1] : 0u;
-  for (uint32_t b = b0; b < b1; ++b) {
-    offsets[b] = run;
-    run += counts[b];
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS)
+k_scan_chunks(const uint32_t* __restrict__ counts, const unsigned* __restrict__ count_ptr,
+              uint32_t* __restrict__ offsets, uint32_t* __restrict__ chunk) {
+  __shared__ uint32_t part[SCAN_THREADS];
+  const uint32_t nblocks = (*count_ptr + BLOCK - 1) / BLOCK;
+  const uint32_t base = blockIdx.x * SCAN_CHUNK;
+  if (base >= nblocks) return;  // whole blocks only
+  const uint32_t tid = threadIdx.x;
+  const uint32_t b0 = base + tid * SCAN_PER;
+  uint32_t v[SCAN_PER];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k) {
+    v[k] = b0 + k < nblocks ? counts[b0 + k] : 0u;
+    sum += v[k];
   }
+  part[tid] = sum;
+  __syncthreads();
+  block_inclusive_scan(part);
+  uint32_t run = tid ? part[tid - 1] : 0u;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k) {
+    if (b0 + k < nblocks) offsets[b0 + k] = run;
+    run += v[k];
+  }
+  if (tid == SCAN_THREADS - 1) chunk[blockIdx.x] = part[SCAN_THREADS - 1];
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS)
+k_scan_top(const unsigned* __restrict__ count_ptr, uint32_t* __restrict__ chunk,
+           unsigned int* total, unsigned long long* segments) {
+  __shared__ uint32_t part[SCAN_THREADS];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nblocks = (*count_ptr + BLOCK - 1) / BLOCK;
+  const uint32_t nchunks = (nblocks + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  if (tid == 0) *segments += *count_ptr;
+  const uint32_t v = tid < nchunks ? chunk[tid] : 0u;
+  part[tid] = v;
+  __syncthreads();
+  block_inclusive_scan(part);
+  if (tid < nchunks) chunk[tid] = part[tid] - v;
   if (tid == SCAN_THREADS - 1) *total = part[SCAN_THREADS - 1];
 }
 
@@ -1204,7 +1240,7 @@ k_scan(const uint32_t* __restrict__ counts, const unsigned* __restrict__ count_p
 __global__ void __launch_bounds__(BLOCK)
 k_scatter(const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
           const uint8_t* __restrict__ emitted, const uint32_t* __restrict__ offsets,
-          uint32_t* __restrict__ next) {
+          const uint32_t* __restrict__ chunk, uint32_t* __restrict__ next) {
   __shared__ uint32_t wcount[WAVES];
   const uint32_t count = *count_ptr;
   if (blockIdx.x * BLOCK >= count) return;  // whole blocks only
@@ -1216,7 +1252,7 @@ k_scatter(const uint32_t* __restrict__ list, const unsigned* __restrict__ count_
   const uint32_t w = threadIdx.x / 64;
   if (lane_id() == 0) wcount[w] = (uint32_t)__popcll(m);
   __syncthreads();
-  uint32_t base = offsets[blockIdx.x];
+  uint32_t base = offsets[blockIdx.x] + chunk[blockIdx.x / SCAN_CHUNK];
   for (uint32_t i = 0; i < w; ++i) base += wcount[i];
   const uint64_t below = m & ((1ull << lane_id()) - 1ull);
   if (pred) next[base + (uint32_t)__popcll(below)] = slot;
@@ -1334,7 +1370,8 @@ struct ssolh_context {
   uint64_t nprim = 0;
   uint32_t stack_depth = MAX_STACK;
   // solve buffers
-  DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, counters, tallies, jobs;
+  DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, scan_chunks, counters,
+      tallies, jobs;
   DevBuf adj_state, adj_in, adj_grad;
   DevBuf trk_vtx, trk_type, trk_seq, trk_count;
   uint64_t trk_paths = 0;           // realisations of the last tracked solve
@@ -1471,7 +1508,8 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_scene(c);
   for (DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
-                    &c->block_emits, &c->block_offsets, &c->counters, &c->tallies, &c->jobs,
+                    &c->block_emits, &c->block_offsets, &c->scan_chunks, &c->counters, &c->tallies,
+                    &c->jobs,
                     &c->adj_state, &c->adj_in, &c->adj_grad, &c->trk_vtx, &c->trk_type,
                     &c->trk_seq, &c->trk_count})
     if (b->p) (void)hipFree(b->p);
@@ -1662,6 +1700,8 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   if ((r = dev_alloc(c->emitted, n))) return r;
   if ((r = dev_alloc(c->block_emits, max_blocks * sizeof(uint32_t)))) return r;
   if ((r = dev_alloc(c->block_offsets, max_blocks * sizeof(uint32_t)))) return r;
+  if ((r = dev_alloc(c->scan_chunks, ((max_blocks + SCAN_CHUNK - 1) / SCAN_CHUNK) * sizeof(uint32_t))))
+    return r;
   if ((r = dev_alloc(c->counters, sizeof(Counters) + 4 * sizeof(unsigned)))) return r;
   // adjoint buffers
   AdjParams A{};
@@ -1776,6 +1816,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   uint8_t* emitted = (uint8_t*)c->emitted.p;
   uint32_t* block_emits = (uint32_t*)c->block_emits.p;
   uint32_t* block_offsets = (uint32_t*)c->block_offsets.p;
+  uint32_t* scan_chunks = (uint32_t*)c->scan_chunks.p;
 
   // one iteration: trace, shade, compaction of list i into list i^1
   auto iteration = [&](int i, uint32_t blocks) -> int {
@@ -1811,20 +1852,33 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
                                                  (RayRec32*)c->rays.p, (const HitRec32*)c->hits.p,
                                                  emitted, block_emits, ctr);
     else
-      if (A.grad)
-        k_shade<true><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
-                                                      (RayRec*)c->rays.p, (const HitRec*)c->hits.p,
-                                                      emitted, block_emits, ctr, A, K);
-      else
-        k_shade<false><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
-                                                       (RayRec*)c->rays.p, (const HitRec*)c->hits.p,
-                                                       emitted, block_emits, ctr, A, K);
+      {
+        RayRec* rr = (RayRec*)c->rays.p;
+        const HitRec* hh = (const HitRec*)c->hits.p;
+        const bool refill = slots < N;
+        if (A.grad && refill)
+          k_shade<true, true><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
+                                                              rr, hh, emitted, block_emits, ctr, A, K);
+        else if (A.grad)
+          k_shade<true, false><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
+                                                               rr, hh, emitted, block_emits, ctr, A, K);
+        else if (refill)
+          k_shade<false, true><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
+                                                               rr, hh, emitted, block_emits, ctr, A, K);
+        else
+          k_shade<false, false><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list,
+                                                                count, rr, hh, emitted, block_emits,
+                                                                ctr, A, K);
+      }
     HIP_TRY(hipGetLastError());
     if (!graph_iters) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-    k_scan<<<1, SCAN_THREADS, 0, c->stream>>>(block_emits, count, block_offsets, cnt + (i ^ 1),
-                                              &ctr->segments);
+    k_scan_chunks<<<(blocks + SCAN_CHUNK - 1) / SCAN_CHUNK, SCAN_THREADS, 0, c->stream>>>(
+        block_emits, count, block_offsets, scan_chunks);
     HIP_TRY(hipGetLastError());
-    k_scatter<<<blocks, BLOCK, 0, c->stream>>>(list, count, emitted, block_offsets, lists[i ^ 1]);
+    k_scan_top<<<1, SCAN_THREADS, 0, c->stream>>>(count, scan_chunks, cnt + (i ^ 1), &ctr->segments);
+    HIP_TRY(hipGetLastError());
+    k_scatter<<<blocks, BLOCK, 0, c->stream>>>(list, count, emitted, block_offsets, scan_chunks,
+                                               lists[i ^ 1]);
     HIP_TRY(hipGetLastError());
     return 0;
   };
