@@ -702,7 +702,6 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   P.pos_wl[slot] = make_double4(pos.x, pos.y, pos.z, wl);
   P.dir_cos[slot] = make_double4(dir.x, dir.y, dir.z, cosf);
   P.prev_init[slot] = make_double4(w0, w0, w0, w0);
-  P.out_in0[slot] = make_double4(0, 0, 0, 0);
   P.nrm_t[slot] = make_double4(N.x, N.y, N.z, 0);
   P.pend[slot] = make_double4(0, 0, 0, 0);
   P.meta[slot] = make_uint4((uint32_t)path, (uint32_t)(path >> 32), rng.block, PF_ACTIVE);
@@ -739,9 +738,11 @@ __device__ void flush_sampled_count(bool has, int samp_slot, const Tallies& T) {
 
 // First realisations: slot s holds realisation s, s < count. Slot s then
 // runs realisations s + k * slots, k = 1, 2, ... while they are below N.
-__global__ void __launch_bounds__(BLOCK)
+template <bool ADJ>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
 k_generate(DScene S, PathState P, SolveParams prm, Tallies T, RayRec* __restrict__ rays,
-           uint32_t count, AdjParams A, TrackParams K) {
+           uint32_t count, AdjParams A_, TrackParams K) {
+  const AdjParams A = ADJ ? A_ : AdjParams{};
   const uint32_t slot = blockIdx.x * BLOCK + threadIdx.x;
   const bool live = slot < count;
   Emit e;
@@ -832,7 +833,8 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
     double4 pw = P.pos_wl[slot];
     double4 dc = P.dir_cos[slot];
     double4 pv = P.prev_init[slot];
-    double4 oi = P.out_in0[slot];
+    // outgoing weights of the last shaded point: none before the first one
+    double4 oi = (meta.w & PF_STAGE_BOUNCE) ? P.out_in0[slot] : make_double4(0, 0, 0, 0);
     double4 pd = P.pend[slot];
     const uint64_t path = (uint64_t)meta.x | ((uint64_t)meta.y << 32);
     PathRng rng;
@@ -1847,10 +1849,16 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     }
     HIP_TRY(hipGetLastError());
     if (!graph_iters) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-    if (fp32)
-      k_shade32<<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, c->P32, prm, c->T, list, count,
-                                                 (RayRec32*)c->rays.p, (const HitRec32*)c->hits.p,
-                                                 emitted, block_emits, ctr);
+    if (fp32 && slots < N)
+      k_shade32<true><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, c->P32, prm, c->T, list,
+                                                       count, (RayRec32*)c->rays.p,
+                                                       (const HitRec32*)c->hits.p, emitted,
+                                                       block_emits, ctr);
+    else if (fp32)
+      k_shade32<false><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, c->P32, prm, c->T, list,
+                                                        count, (RayRec32*)c->rays.p,
+                                                        (const HitRec32*)c->hits.p, emitted,
+                                                        block_emits, ctr);
     else
       {
         RayRec* rr = (RayRec*)c->rays.p;
@@ -1893,8 +1901,12 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     k_generate32<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
         c->scn, c->shapeq, c->P32, prm, c->T, (RayRec32*)c->rays.p, count);
   else
-    k_generate<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
-        c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A, K);
+    if (A.grad)
+      k_generate<true><<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
+          c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A, K);
+    else
+      k_generate<false><<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
+          c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A, K);
   HIP_TRY(hipGetLastError());
   uint32_t iter = 0;
   if (!graph_iters) {

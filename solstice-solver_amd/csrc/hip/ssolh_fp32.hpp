@@ -382,7 +382,6 @@ __device__ void generate32(const DScene& S, const ShapeQ* SQ, const PathState32&
   P.pos[slot] = make_float4(off.x, off.y, off.z, __uint_as_float(st.instance));
   P.dir_wl[slot] = make_float4(dir.x, dir.y, dir.z, wl);
   P.prev_init[slot] = make_float4(w0, w0, w0, w0);
-  P.out_in0[slot] = make_float4(0, 0, 0, 0);
   P.nrm_t[slot] = make_float4(N.x, N.y, N.z, 0);
   P.pend_cos[slot] = make_float4(0, 0, 0, cosf_);
   P.meta[slot] = make_uint4((uint32_t)path, (uint32_t)(path >> 32), rng.block, PF_ACTIVE);
@@ -450,7 +449,9 @@ __device__ void spill32(double* base, uint64_t key, const float* w, unsigned lon
 }
 
 // Shading in FP32 (reference trace_radiative_path loop body,
-// src/ssol_solver.c:939-1086); same structure as k_shade.
+// src/ssol_solver.c:939-1086); same structure as k_shade (REFILL: slots take
+// further realisations).
+template <bool REFILL>
 __global__ void __launch_bounds__(BLOCK)
 k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams prm, Tallies T,
           const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
@@ -481,7 +482,7 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
     float4 ps = P.pos[slot];
     float4 dw = P.dir_wl[slot];
     float4 pv = P.prev_init[slot];
-    float4 oi = P.out_in0[slot];
+    float4 oi = (meta.w & PF_STAGE_BOUNCE) ? P.out_in0[slot] : make_float4(0, 0, 0, 0);
     float4 pc = P.pend_cos[slot];
     const uint64_t path = (uint64_t)meta.x | ((uint64_t)meta.y << 32);
     PathRng32 rng;
@@ -745,7 +746,7 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
   const uint64_t id = want ? (((uint64_t)P.meta[slot].y << 32) | P.meta[slot].x) + prm.slots : 0;
   int samp_slot = 0;
   bool gen = false;
-  if (want && id < prm.N) {
+  if (REFILL && want && id < prm.N) {
     generate32(S, SQ, P, prm, slot, id, e, samp_slot);
     gen = true;
   } else if (live && fin.kind != FIN_NONE) {
