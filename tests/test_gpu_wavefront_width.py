@@ -1,0 +1,97 @@
+"""The wavefront width changes speed, not results. By default every
+realisation of a solve is in flight at once; with a narrower width
+(ssol_device_set_wavefront_size) slots take further realisations in place,
+which runs the kernels' refill code. Realisation ids are static, so both
+runs follow the same paths; only the order of the FP64 tally additions may
+differ (tolerance 1e-10 relative). Covers the plain, FP32, adjoint,
+tracked and batched solves."""
+import numpy as np
+import pytest
+
+import ref_scenes as R
+import ssol_ctypes as L
+import ssol_scene as S
+
+pytestmark = pytest.mark.gpu
+
+NARROW = 4096  # 32 blocks of 128 slots
+
+
+def _close(a, b, rel=1e-10):
+    return abs(a[0] - b[0]) <= rel * max(abs(a[0]), abs(b[0])) + 1e-9
+
+
+def _same_globals(ea, eb):
+    ga, gb = ea.global_(), eb.global_()
+    for k in ga:
+        assert _close(ga[k], gb[k]), (k, ga[k], gb[k])
+
+
+SCENES = {
+    "solver1": lambda: R.solver1("all")[0],
+    "solver7_beam_down": lambda: R.solver7()[0],
+    "paraboloid_field": lambda: S.config2(0, n_helio=20, radius=150.0, rcv_size=5.0, rcv_cells=8),
+    "flat_field": lambda: S.config3(0, n_helio=200, r_min=50.0, r_max=250.0),
+}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_narrow_wavefront_gives_the_default_result(gpu, name, precision):
+    spec = SCENES[name]()
+    n = 300_000
+    wide = L.Scene(spec, precision=precision)
+    narrow = L.Scene(spec, precision=precision, path_slots=NARROW)
+    ew, en = wide.solve(n, seed=4), narrow.solve(n, seed=4)
+    assert en.stats.iterations > ew.stats.iterations  # the refill did run
+    assert ew.realisation_count == en.realisation_count == n
+    _same_globals(ew, en)
+    for inst in spec.instances:
+        for side in (S.FRONT, S.BACK):
+            if inst.receiver_mask & side:
+                rw, rn = ew.receiver(inst, side), en.receiver(inst, side)
+                for k in rw:
+                    assert _close(rw[k], rn[k]), (k, rw[k], rn[k])
+    wide.close()
+    narrow.close()
+
+
+def test_narrow_wavefront_adjoint(gpu):
+    spec, aim, slope, axes = S.config5(0, n_helio=20, radius=150.0)
+    helios = spec.instances[:-1]
+    wide = L.Scene(spec)
+    narrow = L.Scene(spec, path_slots=NARROW)
+    gw = wide.solve_adjoint(200_000, helios, axes, d_absorbed=1.0, seed=2)
+    gn = narrow.solve_adjoint(200_000, helios, axes, d_absorbed=1.0, seed=2)
+    assert np.allclose(gw, gn, rtol=1e-9, atol=1e-9 * np.abs(gw).max())
+    wide.close()
+    narrow.close()
+
+
+def test_narrow_wavefront_tracked_paths(gpu):
+    spec = R.solver1("all")[0]
+    wide = L.Scene(spec)
+    narrow = L.Scene(spec, path_slots=NARROW)
+    pw = wide.solve(20_000, seed=6, tracker=(2.0, 30.0)).tracked_paths()
+    pn = narrow.solve(20_000, seed=6, tracker=(2.0, 30.0)).tracked_paths()
+    assert len(pw) == len(pn) == 20_000
+    for (tw, vw), (tn, vn) in zip(pw, pn):
+        assert tw == tn and np.array_equal(vw, vn)
+    wide.close()
+    narrow.close()
+
+
+def test_narrow_wavefront_batch(gpu):
+    spec = S.config2(0, n_helio=20, radius=150.0)
+    dirs = [S.sun_dir_from(a, e) for a, e in ((180, 60), (120, 30), (250, 75))]
+    counts = [70_000, 0, 40_000]
+    wide = L.Scene(spec)
+    narrow = L.Scene(spec, path_slots=NARROW)
+    rw, cw = wide.solve_sun_batch(dirs, counts, seed=3)
+    rn, cn = narrow.solve_sun_batch(dirs, counts, seed=3)
+    assert list(cw) == list(cn)
+    for a, b in zip(rw, rn):
+        for k in a:
+            assert _close(a[k], b[k]), (k, a[k], b[k])
+    wide.close()
+    narrow.close()
