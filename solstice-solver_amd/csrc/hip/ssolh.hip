@@ -51,6 +51,8 @@ constexpr int SCAN_THREADS = 1024;   // threads of a scan block
 constexpr uint32_t SENTINEL = 0xFFFFFFFFu;
 constexpr uint32_t LEAF = 0x80000000u;
 constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t TOP_REF = 0x40000000u;  // inner-node reference into the LDS copy
+constexpr int TOP_NODES = 31;              // TLAS nodes staged in LDS (2 KB): 5 levels
 constexpr int MAX_PEND = 8;          // compile-time bound of pending entries
 
 // Per-thread message of the last failing call
@@ -98,6 +100,11 @@ struct DScene {
   const ssolh_object* objects;
   const ssolh_bvh_node* nodes;
   uint32_t tlas_root;
+  // copies of the TLAS's top levels (breadth first, children in the set
+  // referenced as TOP_REF | index), staged in LDS by the FP64 trace kernel
+  const ssolh_bvh_node* top;
+  uint32_t ntop;
+  uint32_t root_ref;         // first node reference of a traversal
   const ssolh_triangle* tris;
   const ssolh_instance* insts;
   const uint32_t* tlas_inst;
@@ -483,8 +490,13 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
         const RayRec* __restrict__ rays, HitRec* __restrict__ hits) {
   __shared__ uint32_t lds_stack[STACK * BLOCK];
+  __shared__ ssolh_bvh_node lds_top[TOP_NODES];
+  const uint32_t count = *count_ptr;
+  if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: the staging syncs
+  for (uint32_t i = threadIdx.x; i < S.ntop; i += BLOCK) lds_top[i] = S.top[i];
+  __syncthreads();
   const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-  if (q >= *count_ptr) return;
+  if (q >= count) return;
 
   const uint32_t slot = slot_of(list, q);
   const RayRec r = rays[slot];
@@ -519,7 +531,7 @@ k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict_
   // nodes together, then process their leaves together, which keeps the
   // node, instance and triangle code paths from interleaving per lane.
   // SENTINEL (all bits set) reads as a leaf reference and ends the descent.
-  uint32_t ref = S.tlas_root;
+  uint32_t ref = S.root_ref;
   bool alive = true;
   auto pop = [&]() -> bool {
     if (sp == 0) return false;
@@ -529,7 +541,7 @@ k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict_
   };
   while (alive) {
     while (!(ref & LEAF)) {
-      const ssolh_bvh_node n = S.nodes[ref];
+      const ssolh_bvh_node n = (ref & TOP_REF) ? lds_top[ref & 0x0FFFFFFFu] : S.nodes[ref];
       const float tmax = ts.tbest < (double)FLT_MAX ? (float)ts.tbest * 1.0000002f + 1e-30f : FLT_MAX;
       float tn0, tn1;
       const bool h0 = n.count[0] != SSOLH_BVH_EMPTY && box_hit(fr, n.lo[0], n.hi[0], ftmin, tmax, tn0);
@@ -1564,6 +1576,33 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   r |= upload(c, s->shapes, s->nshapes, &d.shapes);
   r |= upload(c, s->objects, s->nobjects, &d.objects);
   r |= upload(c, s->nodes, s->nnodes, &d.nodes);
+  {
+    // the TLAS's first TOP_NODES inner nodes, breadth first, with the
+    // references between them rewritten to TOP_REF | index
+    std::vector<ssolh_bvh_node> top;
+    std::vector<uint32_t> order;
+    std::vector<int32_t> where;  // global node -> top index
+    if (s->tlas_root < s->nnodes) {
+      order.push_back(s->tlas_root);
+      for (size_t h = 0; h < order.size() && order.size() < (size_t)TOP_NODES; ++h) {
+        const ssolh_bvh_node& n = s->nodes[order[h]];
+        for (int k = 0; k < 2 && order.size() < (size_t)TOP_NODES; ++k)
+          if (n.count[k] == 0 && n.child[k] < s->nnodes) order.push_back(n.child[k]);
+      }
+    }
+    where.assign(s->nnodes, -1);
+    for (size_t i = 0; i < order.size(); ++i) where[order[i]] = (int32_t)i;
+    for (uint32_t g : order) {
+      ssolh_bvh_node n = s->nodes[g];
+      for (int k = 0; k < 2; ++k)
+        if (n.count[k] == 0 && n.child[k] < s->nnodes && where[n.child[k]] >= 0)
+          n.child[k] = TOP_REF | (uint32_t)where[n.child[k]];
+      top.push_back(n);
+    }
+    r |= upload(c, top.data(), top.size(), &d.top);
+    d.ntop = (uint32_t)top.size();
+    d.root_ref = top.empty() ? s->tlas_root : TOP_REF;
+  }
   r |= upload(c, s->triangles, s->ntriangles, &d.tris);
   r |= upload(c, s->instances, s->ninstances, &d.insts);
   r |= upload(c, s->tlas_instances, s->ninstances, &d.tlas_inst);
@@ -1642,6 +1681,14 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const uint64_t fit = (uint64_t)(total_b / 5 * 2) / per_slot;
     slots64 = std::min<uint64_t>(std::max<uint64_t>(fit, 1u << 20), 1ull << 27);
+    // and within what is free now (this context's own solve buffers count
+    // as free: they are reallocated), when other work holds device memory
+    size_t held = 0;
+    for (const DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
+                            &c->adj_state, &c->trk_seq})
+      held += b->n;
+    const uint64_t fit_free = (uint64_t)((free_b + held) / 10 * 9) / per_slot;
+    slots64 = std::min<uint64_t>(slots64, std::max<uint64_t>(fit_free, 1u << 16));
     // a captured iteration launches the grid of the whole width even when
     // its list is short: graph mode keeps the width moderate
     if (graph_iters) slots64 = std::min<uint64_t>(slots64, 1u << 22);

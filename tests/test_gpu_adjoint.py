@@ -79,13 +79,15 @@ def test_adjoint_map_variance_vs_crn_fd(small_field):
 
 def test_adjoint_vs_oracle_fd(small_field):
     """CRN differences of the oracle (the CPU restatement of the reference
-    solver) on the largest aim gradient of the absorbed flux."""
+    solver) on the largest aim gradient of the absorbed flux: 6 x 8e6 oracle
+    paths on each side resolve the difference to about 0.4 %, against 1.6e7
+    adjoint paths per seed."""
     fld, aim, slope = small_field
-    grads = [H.adjoint(fld, aim, slope, 1.0, None, seed=s) for s in SEEDS]
+    grads = [H.adjoint(fld, aim, slope, 1.0, None, seed=s, n=16_000_000) for s in SEEDS]
     mean = np.mean(grads, axis=0)
     i, k = np.unravel_index(int(np.argmax(np.abs(mean[:, :2]))), mean[:, :2].shape)
-    fd = [H.crn_fd(fld, aim, slope, i, k, H_FD, lambda a, m: a, seed=s, n=2_000_000,
-                   solver="oracle") for s in (21, 22, 23)]
+    fd = [H.crn_fd(fld, aim, slope, i, k, H_FD, lambda a, m: a, seed=s, n=8_000_000,
+                   solver="oracle") for s in range(21, 27)]
     _check(f"oracle d absorbed / d p[{i},{k}]", [g[i, k] for g in grads], fd)
 
 
