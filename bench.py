@@ -36,6 +36,16 @@ BYTES_SHADE_FP64 = 96 + 32 + 96 + 8
 BYTES_TRACE_FP32 = 40 + 16
 BYTES_SHADE_FP32 = 64 + 16 + 64 + 8
 
+# Work started within a few seconds of closing a context that held tens of
+# GB of device memory ran up to 2x slower on MI355X (the kernels' own times
+# unchanged, the wall time not); the legs below wait that long between
+# contexts so that each one times its own work.
+SETTLE_S = 4.0
+
+
+def settle():
+    time.sleep(SETTLE_S)
+
 
 def merge(a, b):
     """Merge two (E, SE, N) estimates (src/test_ssol_solver6.c:219-235)."""
@@ -128,26 +138,31 @@ def other_forms(spec, B, ttt_batch, args):
         print(f"[bench] {name}: {entry}", file=sys.stderr, flush=True)
         del est
         lib.close()
+        settle()
     return out
 
 
-def sweep(n_suns=512, log_min=5.0, log_max=7.0, seq_suns=64, graph=0):
+def sweep(n_suns=512, log_min=5.0, log_max=7.0, seq_suns=64, graph_iterations=16):
     """The 100-paraboloid field under n_suns sun positions with seeded
-    log-uniform realisation counts: one batched solve and, for comparison,
-    the first seq_suns suns as separate ssol_solve calls on the same scene
-    through the public API (each sun change re-flattens the scene, as any
-    caller's per-sun solve would)."""
+    log-uniform realisation counts: one batched solve driven by the host
+    loop, the same batch with graph_iterations wavefront iterations per HIP
+    graph launch and, for comparison, the first seq_suns suns as separate
+    ssol_solve calls on the same scene through the public API (each sun
+    change re-flattens the scene, as any caller's per-sun solve would)."""
     import ctypes as C
     import ssol_ctypes as L
     import ssol_scene as S
     spec, dirs, counts = S.config4(n_suns=n_suns, log_min=log_min, log_max=log_max)
-    lib = L.Scene(spec, graph_iterations=graph)
-    lib.solve_sun_batch(dirs[:4], [10000] * 4, seed=1)  # warm-up
-    t0 = time.perf_counter()
-    res, got = lib.solve_sun_batch(dirs, counts, seed=2)
-    t_batch = time.perf_counter() - t0
-    lib.close()
     n_total = int(sum(int(c) for c in counts))
+    t_batch = {}
+    for graph in (0, graph_iterations):
+        lib = L.Scene(spec, graph_iterations=graph)
+        lib.solve_sun_batch(dirs[:4], [10000] * 4, seed=1)  # warm-up
+        t0 = time.perf_counter()
+        res, got = lib.solve_sun_batch(dirs, counts, seed=2)
+        t_batch[graph] = time.perf_counter() - t0
+        lib.close()
+        settle()
     # sequential: one solve per sun position on one scene
     sl = L.Scene(spec)
     sl.solve(10000, seed=1)  # warm-up
@@ -159,13 +174,19 @@ def sweep(n_suns=512, log_min=5.0, log_max=7.0, seq_suns=64, graph=0):
         n_seq += int(counts[j])
     t_seq = time.perf_counter() - t0
     sl.close()
-    out = {"suns": n_suns, "paths": n_total, "batched_s": round(t_batch, 4),
-           "batched_Mpaths_s": round(n_total / t_batch / 1e6, 2),
-           "graph_iterations": graph,
+    settle()
+    rate = {g: n_total / t / 1e6 for g, t in t_batch.items()}
+    seq_rate = n_seq / t_seq / 1e6
+    out = {"suns": n_suns, "paths": n_total,
+           "batched_s": round(t_batch[0], 4), "batched_Mpaths_s": round(rate[0], 2),
+           "graph_iterations": graph_iterations,
+           "batched_graph_s": round(t_batch[graph_iterations], 4),
+           "batched_graph_Mpaths_s": round(rate[graph_iterations], 2),
            "sequential_sample_suns": seq_suns, "sequential_sample_paths": n_seq,
            "sequential_sample_s": round(t_seq, 4),
-           "sequential_Mpaths_s": round(n_seq / t_seq / 1e6, 2),
-           "speedup_vs_sequential": round((n_total / t_batch) / (n_seq / t_seq), 2)}
+           "sequential_Mpaths_s": round(seq_rate, 2),
+           "speedup_vs_sequential": round(rate[0] / seq_rate, 2),
+           "graph_speedup_vs_sequential": round(rate[graph_iterations] / seq_rate, 2)}
     print(f"[bench] sweep: {out}", file=sys.stderr, flush=True)
     return out
 
@@ -208,6 +229,7 @@ def adjoint_timing(n=20_000_000):
         if kind == "rebuild":
             sc.close()
     lib.close()
+    settle()
     out = {"heliostats": len(spec.instances) - 1, "paths": n, "forward_s": round(t_fwd, 4),
            "adjoint_s": round(t_adj, 4), "adjoint_Mpaths_s": round(n / t_adj / 1e6, 2),
            "grad_norm": float(np.linalg.norm(g)),
@@ -354,6 +376,7 @@ def main(argv=None):
     lib.close()
 
     if not args.no_forms and args.config == "c3":
+        settle()
         out["forms"] = other_forms(spec, B, min(B, 4_000_000), args)
         out["sweep_512_suns"] = sweep()
         out["adjoint_1000_heliostats"] = adjoint_timing()

@@ -23,7 +23,13 @@ HOST_SOURCES = [os.path.join(HERE, "csrc", "host", f) for f in (
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
                "-munsafe-fp-atomics", "-fvisibility=hidden",
                # keep packed-FP32 instructions out of the generated code
-               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               # no machine-level hoisting of loop invariants: it lifts the
+               # FP64 polynomial and Threefry constants of the path kernels
+               # out of their loops and keeps them live across them, which
+               # spills (k_shade 108 -> 60 bytes/lane of scratch, k_generate
+               # 92 -> 0, the looping graph-captured forms 388 -> 72)
+               "-mllvm", "-disable-machine-licm"]
 
 
 def _run(cmd):

@@ -18,8 +18,9 @@
 //               tallies, Russian roulette, point update) and the head of the
 //               next one (extinction, BSDF or virtual pass-through, receiver
 //               tallies); finished realisations flush their pending tallies
-//               with wave-aggregated FP64 atomics and the slot is refilled
-//               with a new realisation in place;
+//               with wave-aggregated FP64 atomics and, when the solve has
+//               more realisations than slots, the slot is refilled with its
+//               next realisation in place;
 //   k_scan_chunks / k_scan_top / k_scatter  order-preserving compaction of
 //               the slots that wrote a ray into the next work list (per-block
 //               counts, two-level scan of the counts, scatter).
@@ -345,9 +346,9 @@ __device__ __forceinline__ unsigned long long wave_count_add(bool pred, unsigned
   return base + (unsigned long long)__popcll(below);
 }
 
-// Number of set predicates in the block, written by thread 0 to
-// out[blockIdx.x]. Every thread of the block must call it.
-__device__ void block_count(bool pred, uint32_t* out) {
+// Number of set predicates in the block, written by thread 0 to out[b]
+// (b: the logical block). Every thread of the block must call it.
+__device__ void block_count(bool pred, uint32_t* out, uint32_t b) {
   __shared__ uint32_t wcount[WAVES];
   const uint64_t m = __ballot(pred);
   if (lane_id() == 0) wcount[threadIdx.x / 64] = (uint32_t)__popcll(m);
@@ -355,8 +356,9 @@ __device__ void block_count(bool pred, uint32_t* out) {
   if (threadIdx.x == 0) {
     uint32_t s = 0;
     for (int w = 0; w < WAVES; ++w) s += wcount[w];
-    out[blockIdx.x] = s;
+    out[b] = s;
   }
+  __syncthreads();  // wcount is reused by the block's next logical block
 }
 
 // -------------------------------------------------------------------------
@@ -485,18 +487,12 @@ __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t is
 // STACK: LDS stack entries per lane, at least the scene's stack_depth (the
 // smaller the stack, the more blocks share a CU). Four waves per SIMD caps
 // the kernel at 128 VGPRs; the few values that spill are cold.
+// Entry q of the work list: traversal, filter, hit record
 template <int STACK>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
-k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
-        const RayRec* __restrict__ rays, HitRec* __restrict__ hits) {
-  __shared__ uint32_t lds_stack[STACK * BLOCK];
-  __shared__ ssolh_bvh_node lds_top[TOP_NODES];
-  const uint32_t count = *count_ptr;
-  if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: the staging syncs
-  for (uint32_t i = threadIdx.x; i < S.ntop; i += BLOCK) lds_top[i] = S.top[i];
-  __syncthreads();
-  const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-  if (q >= count) return;
+__device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __restrict__ list,
+                                          uint32_t q, const RayRec* __restrict__ rays,
+                                          HitRec* __restrict__ hits, uint32_t* lds_stack,
+                                          const ssolh_bvh_node* lds_top) {
 
   const uint32_t slot = slot_of(list, q);
   const RayRec r = rays[slot];
@@ -622,6 +618,30 @@ k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict_
   h.iss = ts.iss;
   h.prim = ts.prim;
   hits[slot] = h;
+}
+
+// LOOP: graph-captured launch of a fixed grid, each block taking every
+// gridDim.x-th logical block of BLOCK list entries; else one logical block
+// per block
+template <int STACK, bool LOOP>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
+        const RayRec* __restrict__ rays, HitRec* __restrict__ hits) {
+  __shared__ uint32_t lds_stack[STACK * BLOCK];
+  __shared__ ssolh_bvh_node lds_top[TOP_NODES];
+  const uint32_t count = *count_ptr;
+  if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: the staging syncs
+  for (uint32_t i = threadIdx.x; i < S.ntop; i += BLOCK) lds_top[i] = S.top[i];
+  __syncthreads();
+  if (LOOP) {
+    for (uint32_t b = blockIdx.x; b * BLOCK < count; b += gridDim.x) {
+      const uint32_t q = b * BLOCK + threadIdx.x;
+      if (q < count) trace_ray<STACK>(S, list, q, rays, hits, lds_stack, lds_top);
+    }
+  } else {
+    const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
+    if (q < count) trace_ray<STACK>(S, list, q, rays, hits, lds_stack, lds_top);
+  }
 }
 
 // -------------------------------------------------------------------------
@@ -811,19 +831,18 @@ __device__ void spill(double* base, uint64_t key, const double* w, unsigned long
   }
 }
 
-// ADJ: the adjoint solve's variant; the plain one carries no score code.
-// REFILL: slots take further realisations (fewer slots than realisations);
-// without it the kernel carries no generation code.
+// Logical block b (list entries b*BLOCK ...) of k_shade; ADJ: the adjoint
+// solve's variant, the plain one carries no score code; REFILL: slots take
+// further realisations (fewer slots than realisations), without it no
+// generation code.
 template <bool ADJ, bool REFILL>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
-k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __restrict__ list,
-        const unsigned* __restrict__ count_ptr, RayRec* __restrict__ rays,
-        const HitRec* __restrict__ hits, uint8_t* __restrict__ emitted,
-        uint32_t* __restrict__ block_emits, Counters* C, AdjParams A_, TrackParams K) {
+__device__ __forceinline__ void shade_block(
+    uint32_t b, const DScene& S, const PathState& P, const SolveParams& prm, const Tallies& T,
+    const uint32_t* __restrict__ list, uint32_t count, RayRec* __restrict__ rays,
+    const HitRec* __restrict__ hits, uint8_t* __restrict__ emitted,
+    uint32_t* __restrict__ block_emits, Counters* C, const AdjParams& A_, const TrackParams& K) {
   const AdjParams A = ADJ ? A_ : AdjParams{};
-  const uint32_t count = *count_ptr;
-  if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: block_count syncs
-  const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
+  const uint32_t q = b * BLOCK + threadIdx.x;
   const bool live = q < count;
   const size_t stride = prm.slots;
 
@@ -1180,7 +1199,55 @@ k_shade(DScene S, PathState P, SolveParams prm, Tallies T, const uint32_t* __res
   // --- the slot's next ray and its place in the next work list -----------
   if (e.has) rays[slot] = e.r;
   if (live) emitted[slot] = e.has ? 1 : 0;
-  block_count(e.has, block_emits);
+  block_count(e.has, block_emits, b);
+}
+
+// The arguments of a shade kernel in one struct: the struct is the kernel's
+// argument segment (offset 0), which a looping launch reads again for each
+// logical block (kernel_args)
+struct ShadeArgs {
+  DScene S;
+  PathState P;
+  SolveParams prm;
+  Tallies T;
+  const uint32_t* list;
+  const unsigned* count;
+  RayRec* rays;
+  const HitRec* hits;
+  uint8_t* emitted;
+  uint32_t* block_emits;
+  Counters* C;
+  AdjParams A;
+  TrackParams K;
+};
+
+// The kernel's argument segment as an Args, behind a pointer the compiler
+// cannot see through. A looping kernel takes it anew for each logical
+// block, so the scalar loads of the arguments stay next to their uses: hoisted
+// out of the loop, the dozens of argument words would be held in scalar
+// registers across it and spill (through vector registers into scratch).
+template <typename Args>
+__device__ __forceinline__ const Args& kernel_args() {
+  auto p = (const __attribute__((address_space(4))) Args*)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return *(const Args*)p;
+}
+
+template <bool ADJ, bool REFILL, bool LOOP>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
+k_shade(ShadeArgs a) {
+  const uint32_t count = *a.count;
+  // whole logical blocks only: block_count syncs (LOOP: as in k_trace)
+  if (LOOP) {
+    for (uint32_t b = blockIdx.x; b * BLOCK < count; b += gridDim.x) {
+      const ShadeArgs& f = kernel_args<ShadeArgs>();
+      shade_block<ADJ, REFILL>(b, f.S, f.P, f.prm, f.T, f.list, count, f.rays, f.hits, f.emitted,
+                               f.block_emits, f.C, f.A, f.K);
+    }
+  } else if (blockIdx.x * BLOCK < count) {
+    shade_block<ADJ, REFILL>(blockIdx.x, a.S, a.P, a.prm, a.T, a.list, count, a.rays, a.hits,
+                             a.emitted, a.block_emits, a.C, a.A, a.K);
+  }
 }
 
 // -------------------------------------------------------------------------
@@ -1251,25 +1318,29 @@ k_scan_top(const unsigned* __restrict__ count_ptr, uint32_t* __restrict__ chunk,
 
 // Entry q of the next list: the q-th slot (in list order) that wrote a ray.
 // Same block partition as k_shade.
+template <bool LOOP>
 __global__ void __launch_bounds__(BLOCK)
 k_scatter(const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
           const uint8_t* __restrict__ emitted, const uint32_t* __restrict__ offsets,
           const uint32_t* __restrict__ chunk, uint32_t* __restrict__ next) {
   __shared__ uint32_t wcount[WAVES];
   const uint32_t count = *count_ptr;
-  if (blockIdx.x * BLOCK >= count) return;  // whole blocks only
-  const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-  const bool live = q < count;
-  const uint32_t slot = live ? slot_of(list, q) : 0;
-  const bool pred = live && emitted[slot];
-  const uint64_t m = __ballot(pred);
-  const uint32_t w = threadIdx.x / 64;
-  if (lane_id() == 0) wcount[w] = (uint32_t)__popcll(m);
-  __syncthreads();
-  uint32_t base = offsets[blockIdx.x] + chunk[blockIdx.x / SCAN_CHUNK];
-  for (uint32_t i = 0; i < w; ++i) base += wcount[i];
-  const uint64_t below = m & ((1ull << lane_id()) - 1ull);
-  if (pred) next[base + (uint32_t)__popcll(below)] = slot;
+  const uint32_t b_end = LOOP ? count : min(count, (blockIdx.x + 1) * BLOCK);
+  for (uint32_t b = blockIdx.x; b * BLOCK < b_end; b += gridDim.x) {  // whole logical blocks
+    const uint32_t q = b * BLOCK + threadIdx.x;
+    const bool live = q < count;
+    const uint32_t slot = live ? slot_of(list, q) : 0;
+    const bool pred = live && emitted[slot];
+    const uint64_t m = __ballot(pred);
+    const uint32_t w = threadIdx.x / 64;
+    if (lane_id() == 0) wcount[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = offsets[b] + chunk[b / SCAN_CHUNK];
+    for (uint32_t i = 0; i < w; ++i) base += wcount[i];
+    const uint64_t below = m & ((1ull << lane_id()) - 1ull);
+    if (pred) next[base + (uint32_t)__popcll(below)] = slot;
+    __syncthreads();  // wcount is reused by the next logical block
+  }
 }
 
 #include "ssolh_fp32.hpp"
@@ -1403,9 +1474,40 @@ struct ssolh_context {
   uint64_t segments = 0;
   double kernel_ms = 0, trace_ms = 0, shade_ms = 0;
   uint32_t iterations = 0;
+  // grid of each looping kernel: its workgroups resident at once
+  std::vector<std::pair<const void*, uint32_t>> resident;
 };
 
 namespace {
+
+// Workgroups of each looping (graph-captured) kernel resident on the device
+// at once: occupancy per CU x CUs. A looping launch uses that grid, so every
+// workgroup starts at once and takes an equal share of the logical blocks;
+// with a larger grid the workgroups past the resident ones wait for whole
+// shares of others to end, and the tail runs on part of the chip.
+int init_resident_grids(ssolh_context* c, uint32_t cus) {
+  const void* fns[] = {
+      (const void*)k_trace<16, true>, (const void*)k_trace<24, true>,
+      (const void*)k_trace<32, true>, (const void*)k_trace<MAX_STACK, true>,
+      (const void*)k_trace32<16, true>, (const void*)k_trace32<24, true>,
+      (const void*)k_trace32<32, true>, (const void*)k_trace32<MAX_STACK, true>,
+      (const void*)k_shade<false, false, true>, (const void*)k_shade<false, true, true>,
+      (const void*)k_shade<true, false, true>, (const void*)k_shade<true, true, true>,
+      (const void*)k_shade32<false, true>, (const void*)k_shade32<true, true>,
+      (const void*)k_scatter<true>};
+  for (const void* f : fns) {
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, BLOCK, 0));
+    c->resident.push_back({f, (uint32_t)std::max(per_cu, 1) * cus});
+  }
+  return 0;
+}
+
+uint32_t resident_grid(const ssolh_context* c, const void* f) {
+  for (const auto& e : c->resident)
+    if (e.first == f) return e.second;
+  return 1024;  // not reached: init_resident_grids lists every looping kernel
+}
 
 int dev_alloc(DevBuf& b, size_t bytes) {
   if (b.n >= bytes && b.p) return 0;
@@ -1512,6 +1614,10 @@ SSOLH_API int ssolh_create(int device, ssolh_context** out) {
   HIP_TRY(hipSetDevice(device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
+  if (int r = init_resident_grids(c, (uint32_t)prop.multiProcessorCount)) {
+    ssolh_destroy(c);
+    return r;
+  }
   *out = c;
   return 0;
 }
@@ -1689,9 +1795,6 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       held += b->n;
     const uint64_t fit_free = (uint64_t)((free_b + held) / 10 * 9) / per_slot;
     slots64 = std::min<uint64_t>(slots64, std::max<uint64_t>(fit_free, 1u << 16));
-    // a captured iteration launches the grid of the whole width even when
-    // its list is short: graph mode keeps the width moderate
-    if (graph_iters) slots64 = std::min<uint64_t>(slots64, 1u << 22);
   }
   if (slots64 > N) slots64 = N;
   slots64 = (slots64 + BLOCK - 1) / BLOCK * BLOCK;
@@ -1869,62 +1972,75 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
 
   // one iteration: trace, shade, compaction of list i into list i^1
   auto iteration = [&](int i, uint32_t blocks) -> int {
+    // blocks: logical blocks of the list (of its largest length under graph
+    // capture). The host loop launches one workgroup per logical block; a
+    // captured iteration launches each kernel's resident grid, whose
+    // workgroups loop over the logical blocks, however long or short the
+    // list turns out to be
+    const bool loop = graph_iters != 0;
+    auto grid_of = [&](const void* f) {
+      return loop ? std::min<uint32_t>(blocks, resident_grid(c, f)) : blocks;
+    };
     const uint32_t* list = lists[i];
     const unsigned* count = cnt + i;
+    const bool refill = slots < N;
     if (fp32) {
       RayRec32* r32 = (RayRec32*)c->rays.p;
       HitRec32* h32 = (HitRec32*)c->hits.p;
+      auto launch = [&](auto kernel) {
+        kernel<<<grid_of((const void*)kernel), BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list,
+                                                                     count, r32, h32);
+      };
       if (c->stack_depth <= 16)
-        k_trace32<16><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list, count, r32, h32);
+        loop ? launch(k_trace32<16, true>) : launch(k_trace32<16, false>);
       else if (c->stack_depth <= 24)
-        k_trace32<24><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list, count, r32, h32);
+        loop ? launch(k_trace32<24, true>) : launch(k_trace32<24, false>);
       else if (c->stack_depth <= 32)
-        k_trace32<32><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list, count, r32, h32);
+        loop ? launch(k_trace32<32, true>) : launch(k_trace32<32, false>);
       else
-        k_trace32<MAX_STACK><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list, count, r32, h32);
+        loop ? launch(k_trace32<MAX_STACK, true>) : launch(k_trace32<MAX_STACK, false>);
     } else {
       RayRec* r64 = (RayRec*)c->rays.p;
       HitRec* h64 = (HitRec*)c->hits.p;
+      auto launch = [&](auto kernel) {
+        kernel<<<grid_of((const void*)kernel), BLOCK, 0, c->stream>>>(c->scn, list, count, r64,
+                                                                     h64);
+      };
       if (c->stack_depth <= 16)
-        k_trace<16><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, r64, h64);
+        loop ? launch(k_trace<16, true>) : launch(k_trace<16, false>);
       else if (c->stack_depth <= 24)
-        k_trace<24><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, r64, h64);
+        loop ? launch(k_trace<24, true>) : launch(k_trace<24, false>);
       else if (c->stack_depth <= 32)
-        k_trace<32><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, r64, h64);
+        loop ? launch(k_trace<32, true>) : launch(k_trace<32, false>);
       else
-        k_trace<MAX_STACK><<<blocks, BLOCK, 0, c->stream>>>(c->scn, list, count, r64, h64);
+        loop ? launch(k_trace<MAX_STACK, true>) : launch(k_trace<MAX_STACK, false>);
     }
     HIP_TRY(hipGetLastError());
     if (!graph_iters) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-    if (fp32 && slots < N)
-      k_shade32<true><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, c->P32, prm, c->T, list,
-                                                       count, (RayRec32*)c->rays.p,
-                                                       (const HitRec32*)c->hits.p, emitted,
-                                                       block_emits, ctr);
-    else if (fp32)
-      k_shade32<false><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, c->P32, prm, c->T, list,
-                                                        count, (RayRec32*)c->rays.p,
-                                                        (const HitRec32*)c->hits.p, emitted,
-                                                        block_emits, ctr);
-    else
-      {
-        RayRec* rr = (RayRec*)c->rays.p;
-        const HitRec* hh = (const HitRec*)c->hits.p;
-        const bool refill = slots < N;
-        if (A.grad && refill)
-          k_shade<true, true><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
-                                                              rr, hh, emitted, block_emits, ctr, A, K);
-        else if (A.grad)
-          k_shade<true, false><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
-                                                               rr, hh, emitted, block_emits, ctr, A, K);
-        else if (refill)
-          k_shade<false, true><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list, count,
-                                                               rr, hh, emitted, block_emits, ctr, A, K);
-        else
-          k_shade<false, false><<<blocks, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, list,
-                                                                count, rr, hh, emitted, block_emits,
-                                                                ctr, A, K);
-      }
+    if (fp32) {
+      const Shade32Args args{c->scn,     c->shapeq, c->P32, prm, c->T, list, count,
+                             (RayRec32*)c->rays.p, (const HitRec32*)c->hits.p, emitted,
+                             block_emits, ctr};
+      auto launch = [&](auto kernel) {
+        kernel<<<grid_of((const void*)kernel), BLOCK, 0, c->stream>>>(args);
+      };
+      if (refill) loop ? launch(k_shade32<true, true>) : launch(k_shade32<true, false>);
+      else loop ? launch(k_shade32<false, true>) : launch(k_shade32<false, false>);
+    } else {
+      const ShadeArgs args{c->scn,  c->P,     prm,         c->T, list, count, (RayRec*)c->rays.p,
+                           (const HitRec*)c->hits.p, emitted, block_emits, ctr,  A,    K};
+      auto launch = [&](auto kernel) {
+        kernel<<<grid_of((const void*)kernel), BLOCK, 0, c->stream>>>(args);
+      };
+      if (A.grad && refill)
+        loop ? launch(k_shade<true, true, true>) : launch(k_shade<true, true, false>);
+      else if (A.grad)
+        loop ? launch(k_shade<true, false, true>) : launch(k_shade<true, false, false>);
+      else if (refill)
+        loop ? launch(k_shade<false, true, true>) : launch(k_shade<false, true, false>);
+      else
+        loop ? launch(k_shade<false, false, true>) : launch(k_shade<false, false, false>);
+    }
     HIP_TRY(hipGetLastError());
     if (!graph_iters) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
     k_scan_chunks<<<(blocks + SCAN_CHUNK - 1) / SCAN_CHUNK, SCAN_THREADS, 0, c->stream>>>(
@@ -1932,8 +2048,12 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     HIP_TRY(hipGetLastError());
     k_scan_top<<<1, SCAN_THREADS, 0, c->stream>>>(count, scan_chunks, cnt + (i ^ 1), &ctr->segments);
     HIP_TRY(hipGetLastError());
-    k_scatter<<<blocks, BLOCK, 0, c->stream>>>(list, count, emitted, block_offsets, scan_chunks,
-                                               lists[i ^ 1]);
+    if (loop)
+      k_scatter<true><<<grid_of((const void*)k_scatter<true>), BLOCK, 0, c->stream>>>(
+          list, count, emitted, block_offsets, scan_chunks, lists[i ^ 1]);
+    else
+      k_scatter<false><<<blocks, BLOCK, 0, c->stream>>>(list, count, emitted, block_offsets,
+                                                        scan_chunks, lists[i ^ 1]);
     HIP_TRY(hipGetLastError());
     return 0;
   };

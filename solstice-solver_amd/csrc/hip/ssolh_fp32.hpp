@@ -164,14 +164,12 @@ __device__ bool filter32(const DScene& S, const ShapeQ* SQ, const RayRec32& r, f
   return true;
 }
 
+// Entry q of the work list, FP32 form
 template <int STACK>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
-k_trace32(DScene S, const ShapeQ* __restrict__ SQ, const uint32_t* __restrict__ list,
-          const unsigned* __restrict__ count_ptr, const RayRec32* __restrict__ rays,
-          HitRec32* __restrict__ hits) {
-  __shared__ uint32_t lds_stack[STACK * BLOCK];
-  const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-  if (q >= *count_ptr) return;
+__device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __restrict__ SQ,
+                                            const uint32_t* __restrict__ list, uint32_t q,
+                                            const RayRec32* __restrict__ rays,
+                                            HitRec32* __restrict__ hits, uint32_t* lds_stack) {
 
   const uint32_t slot = slot_of(list, q);
   const RayRec32 r = rays[slot];
@@ -320,6 +318,24 @@ k_trace32(DScene S, const ShapeQ* __restrict__ SQ, const uint32_t* __restrict__ 
   hits[slot] = h;
 }
 
+template <int STACK, bool LOOP>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_trace32(DScene S, const ShapeQ* __restrict__ SQ, const uint32_t* __restrict__ list,
+          const unsigned* __restrict__ count_ptr, const RayRec32* __restrict__ rays,
+          HitRec32* __restrict__ hits) {
+  __shared__ uint32_t lds_stack[STACK * BLOCK];
+  const uint32_t count = *count_ptr;
+  if (LOOP) {  // graph-captured launch of a fixed grid (as k_trace)
+    for (uint32_t b = blockIdx.x; b * BLOCK < count; b += gridDim.x) {
+      const uint32_t q = b * BLOCK + threadIdx.x;
+      if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack);
+    }
+  } else {
+    const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
+    if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack);
+  }
+}
+
 // Path generation in FP32 (reference point_init, src/ssol_solver.c:207-358)
 struct Emit32 {
   bool has;
@@ -449,17 +465,15 @@ __device__ void spill32(double* base, uint64_t key, const float* w, unsigned lon
 }
 
 // Shading in FP32 (reference trace_radiative_path loop body,
-// src/ssol_solver.c:939-1086); same structure as k_shade (REFILL: slots take
-// further realisations).
+// src/ssol_solver.c:939-1086), logical block b; same structure as k_shade
+// (REFILL: slots take further realisations).
 template <bool REFILL>
-__global__ void __launch_bounds__(BLOCK)
-k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams prm, Tallies T,
-          const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
-          RayRec32* __restrict__ rays, const HitRec32* __restrict__ hits,
-          uint8_t* __restrict__ emitted, uint32_t* __restrict__ block_emits, Counters* C) {
-  const uint32_t count = *count_ptr;
-  if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: block_count syncs
-  const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void shade32_block(
+    uint32_t b, const DScene& S, const ShapeQ* __restrict__ SQ, const PathState32& P,
+    const SolveParams& prm, const Tallies& T, const uint32_t* __restrict__ list, uint32_t count,
+    RayRec32* __restrict__ rays, const HitRec32* __restrict__ hits,
+    uint8_t* __restrict__ emitted, uint32_t* __restrict__ block_emits, Counters* C) {
+  const uint32_t q = b * BLOCK + threadIdx.x;
   const bool live = q < count;
   const size_t stride = prm.slots;
 
@@ -756,5 +770,37 @@ k_shade32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams pr
 
   if (e.has) rays[slot] = e.r;
   if (live) emitted[slot] = e.has ? 1 : 0;
-  block_count(e.has, block_emits);
+  block_count(e.has, block_emits, b);
+}
+
+// The arguments of k_shade32 in one struct (as ShadeArgs)
+struct Shade32Args {
+  DScene S;
+  const ShapeQ* SQ;
+  PathState32 P;
+  SolveParams prm;
+  Tallies T;
+  const uint32_t* list;
+  const unsigned* count;
+  RayRec32* rays;
+  const HitRec32* hits;
+  uint8_t* emitted;
+  uint32_t* block_emits;
+  Counters* C;
+};
+
+template <bool REFILL, bool LOOP>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
+k_shade32(Shade32Args a) {
+  const uint32_t count = *a.count;
+  if (LOOP) {  // whole logical blocks, arguments read per block (as k_shade)
+    for (uint32_t b = blockIdx.x; b * BLOCK < count; b += gridDim.x) {
+      const Shade32Args& f = kernel_args<Shade32Args>();
+      shade32_block<REFILL>(b, f.S, f.SQ, f.P, f.prm, f.T, f.list, count, f.rays, f.hits,
+                            f.emitted, f.block_emits, f.C);
+    }
+  } else if (blockIdx.x * BLOCK < count) {
+    shade32_block<REFILL>(blockIdx.x, a.S, a.SQ, a.P, a.prm, a.T, a.list, count, a.rays, a.hits,
+                          a.emitted, a.block_emits, a.C);
+  }
 }
