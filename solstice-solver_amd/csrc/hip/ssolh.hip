@@ -535,25 +535,24 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
     ref = stack[sp * BLOCK];
     return true;
   };
+  // box culling distance: the best hit so far, widened by 2 ulp (FP32)
+  float tmax = FLT_MAX;
   while (alive) {
     while (!(ref & LEAF)) {
       const ssolh_bvh_node n = (ref & TOP_REF) ? lds_top[ref & 0x0FFFFFFFu] : S.nodes[ref];
-      const float tmax = ts.tbest < (double)FLT_MAX ? (float)ts.tbest * 1.0000002f + 1e-30f : FLT_MAX;
       float tn0, tn1;
       const bool h0 = n.count[0] != SSOLH_BVH_EMPTY && box_hit(fr, n.lo[0], n.hi[0], ftmin, tmax, tn0);
       const bool h1 = n.count[1] != SSOLH_BVH_EMPTY && box_hit(fr, n.lo[1], n.hi[1], ftmin, tmax, tn1);
-      auto enc = [&](int c) -> uint32_t {
-        return n.count[c] == 0 ? n.child[c] : (LEAF | ((n.count[c] - 1u) << 28) | n.child[c]);
-      };
+      // child references come encoded (leaf bit, count, first item)
       if (h0 && h1) {
         const bool near0 = tn0 <= tn1;
-        const uint32_t rn = enc(near0 ? 0 : 1), rf = enc(near0 ? 1 : 0);
+        const uint32_t rn = near0 ? n.child[0] : n.child[1], rf = near0 ? n.child[1] : n.child[0];
         if (sp < STACK) { stack[sp * BLOCK] = rf; ++sp; }
         ref = rn;
       } else if (h0) {
-        ref = enc(0);
+        ref = n.child[0];
       } else if (h1) {
-        ref = enc(1);
+        ref = n.child[1];
       } else if (!pop()) {
         alive = false;
         break;
@@ -600,6 +599,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
       const uint32_t iss = iss_first + tri.sshape;
       if (!filter(S, r, ow, dw, iss, tri, t, ts, ng, punched)) continue;
       ts.tbest = t;
+      tmax = t < (double)FLT_MAX ? (float)t * 1.0000002f + 1e-30f : FLT_MAX;
       ts.iss = iss;
       ts.prim = tri.prim;
       ts.ng = ng;
@@ -1681,7 +1681,30 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   r |= upload(c, s->materials, s->nmaterials, &d.mtls);
   r |= upload(c, s->shapes, s->nshapes, &d.shapes);
   r |= upload(c, s->objects, s->nobjects, &d.objects);
-  r |= upload(c, s->nodes, s->nnodes, &d.nodes);
+  // Device copy of the nodes with each child reference in the form the
+  // traversal follows: an inner child stays its node index, a leaf becomes
+  // LEAF | (count - 1) << 28 | first item (count stays; SSOLH_BVH_EMPTY
+  // still marks an empty child)
+  std::vector<ssolh_bvh_node> dnodes(s->nodes, s->nodes + s->nnodes);
+  if (s->nnodes >= TOP_REF) { set_err("ssolh_scene_upload: %u BVH nodes", s->nnodes); return SSOLH_ERR_LIMIT; }
+  for (ssolh_bvh_node& n : dnodes)
+    for (int k = 0; k < 2; ++k) {
+      if (n.count[k] == SSOLH_BVH_EMPTY) continue;
+      if (n.count[k] == 0) {
+        if (n.child[k] >= s->nnodes) {
+          set_err("ssolh_scene_upload: BVH child %u out of %u nodes", n.child[k], s->nnodes);
+          return SSOLH_ERR_ARG;
+        }
+        continue;
+      }
+      if (n.count[k] > 8 || n.child[k] >= (1u << 28)) {
+        set_err("ssolh_scene_upload: BVH leaf of %u items at %u (at most 8 below 2^28)",
+                n.count[k], n.child[k]);
+        return SSOLH_ERR_LIMIT;
+      }
+      n.child[k] = LEAF | ((n.count[k] - 1u) << 28) | n.child[k];
+    }
+  r |= upload(c, dnodes.data(), dnodes.size(), &d.nodes);
   {
     // the TLAS's first TOP_NODES inner nodes, breadth first, with the
     // references between them rewritten to TOP_REF | index
@@ -1691,17 +1714,17 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
     if (s->tlas_root < s->nnodes) {
       order.push_back(s->tlas_root);
       for (size_t h = 0; h < order.size() && order.size() < (size_t)TOP_NODES; ++h) {
-        const ssolh_bvh_node& n = s->nodes[order[h]];
+        const ssolh_bvh_node& n = dnodes[order[h]];
         for (int k = 0; k < 2 && order.size() < (size_t)TOP_NODES; ++k)
-          if (n.count[k] == 0 && n.child[k] < s->nnodes) order.push_back(n.child[k]);
+          if (n.count[k] == 0) order.push_back(n.child[k]);
       }
     }
     where.assign(s->nnodes, -1);
     for (size_t i = 0; i < order.size(); ++i) where[order[i]] = (int32_t)i;
     for (uint32_t g : order) {
-      ssolh_bvh_node n = s->nodes[g];
+      ssolh_bvh_node n = dnodes[g];
       for (int k = 0; k < 2; ++k)
-        if (n.count[k] == 0 && n.child[k] < s->nnodes && where[n.child[k]] >= 0)
+        if (n.count[k] == 0 && where[n.child[k]] >= 0)
           n.child[k] = TOP_REF | (uint32_t)where[n.child[k]];
       top.push_back(n);
     }

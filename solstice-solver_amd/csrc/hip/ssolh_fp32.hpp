@@ -213,25 +213,22 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
     ref = stack[sp * BLOCK];
     return true;
   };
+  float tmax = FLT_MAX;  // box culling distance (as trace_ray)
   while (alive) {
     while (!(ref & LEAF)) {
       const ssolh_bvh_node n = S.nodes[ref];
-      const float tmax = ts.tbest < FLT_MAX ? ts.tbest * 1.0000002f + 1e-30f : FLT_MAX;
       float tn0, tn1;
       const bool h0 = n.count[0] != SSOLH_BVH_EMPTY && box_hit(fr, n.lo[0], n.hi[0], ftmin, tmax, tn0);
       const bool h1 = n.count[1] != SSOLH_BVH_EMPTY && box_hit(fr, n.lo[1], n.hi[1], ftmin, tmax, tn1);
-      auto enc = [&](int c) -> uint32_t {
-        return n.count[c] == 0 ? n.child[c] : (LEAF | ((n.count[c] - 1u) << 28) | n.child[c]);
-      };
       if (h0 && h1) {
         const bool near0 = tn0 <= tn1;
-        const uint32_t rn = enc(near0 ? 0 : 1), rf = enc(near0 ? 1 : 0);
+        const uint32_t rn = near0 ? n.child[0] : n.child[1], rf = near0 ? n.child[1] : n.child[0];
         if (sp < STACK) { stack[sp * BLOCK] = rf; ++sp; }
         ref = rn;
       } else if (h0) {
-        ref = enc(0);
+        ref = n.child[0];
       } else if (h1) {
-        ref = enc(1);
+        ref = n.child[1];
       } else if (!pop()) {
         alive = false;
         break;
@@ -274,6 +271,7 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
       const uint32_t iss = iss_first + tri.sshape;
       if (!filter32(S, SQ, r, o, d, iss, tri, t, ts, nobj, punched)) continue;
       ts.tbest = t;
+      tmax = t < FLT_MAX ? t * 1.0000002f + 1e-30f : FLT_MAX;
       ts.iss = iss;
       ts.prim = tri.prim;
       ts.nobj = nobj;
