@@ -484,6 +484,18 @@ __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t is
   return true;
 }
 
+// Node i of the LDS copy of the top levels, read through an LDS (address
+// space 3) pointer: the load is a ds_read, where a pointer that may address
+// either LDS or global memory would make it a flat access
+__device__ __forceinline__ ssolh_bvh_node lds_node(const ssolh_bvh_node* lds, uint32_t i) {
+  typedef const __attribute__((address_space(3))) uint32_t* lds_u32;
+  const lds_u32 p = (lds_u32)(lds + i);
+  ssolh_bvh_node n;
+  uint32_t* w = (uint32_t*)&n;
+  for (int k = 0; k < (int)(sizeof(n) / 4); ++k) w[k] = p[k];
+  return n;
+}
+
 // STACK: LDS stack entries per lane, at least the scene's stack_depth (the
 // smaller the stack, the more blocks share a CU). Four waves per SIMD caps
 // the kernel at 128 VGPRs; the few values that spill are cold.
@@ -539,10 +551,18 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
   float tmax = FLT_MAX;
   while (alive) {
     while (!(ref & LEAF)) {
-      const ssolh_bvh_node n = (ref & TOP_REF) ? lds_top[ref & 0x0FFFFFFFu] : S.nodes[ref];
+      // the whole node in one round of loads (LDS or global memory, each
+      // through its own address space), then both boxes
+      ssolh_bvh_node n;
+      if (ref & TOP_REF)
+        n = lds_node(lds_top, ref & 0x0FFFFFFFu);
+      else
+        n = S.nodes[ref];
       float tn0, tn1;
-      const bool h0 = n.count[0] != SSOLH_BVH_EMPTY && box_hit(fr, n.lo[0], n.hi[0], ftmin, tmax, tn0);
-      const bool h1 = n.count[1] != SSOLH_BVH_EMPTY && box_hit(fr, n.lo[1], n.hi[1], ftmin, tmax, tn1);
+      const bool b0 = box_hit(fr, n.lo[0], n.hi[0], ftmin, tmax, tn0);
+      const bool b1 = box_hit(fr, n.lo[1], n.hi[1], ftmin, tmax, tn1);
+      const bool h0 = b0 & (n.count[0] != SSOLH_BVH_EMPTY);
+      const bool h1 = b1 & (n.count[1] != SSOLH_BVH_EMPTY);
       // child references come encoded (leaf bit, count, first item)
       if (h0 && h1) {
         const bool near0 = tn0 <= tn1;

@@ -216,10 +216,12 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
   float tmax = FLT_MAX;  // box culling distance (as trace_ray)
   while (alive) {
     while (!(ref & LEAF)) {
-      const ssolh_bvh_node n = S.nodes[ref];
+      const ssolh_bvh_node n = S.nodes[ref];  // one round of loads, then both boxes
       float tn0, tn1;
-      const bool h0 = n.count[0] != SSOLH_BVH_EMPTY && box_hit(fr, n.lo[0], n.hi[0], ftmin, tmax, tn0);
-      const bool h1 = n.count[1] != SSOLH_BVH_EMPTY && box_hit(fr, n.lo[1], n.hi[1], ftmin, tmax, tn1);
+      const bool b0 = box_hit(fr, n.lo[0], n.hi[0], ftmin, tmax, tn0);
+      const bool b1 = box_hit(fr, n.lo[1], n.hi[1], ftmin, tmax, tn1);
+      const bool h0 = b0 & (n.count[0] != SSOLH_BVH_EMPTY);
+      const bool h1 = b1 & (n.count[1] != SSOLH_BVH_EMPTY);
       if (h0 && h1) {
         const bool near0 = tn0 <= tn1;
         const uint32_t rn = near0 ? n.child[0] : n.child[1], rf = near0 ? n.child[1] : n.child[0];
