@@ -435,10 +435,16 @@ struct TraceState {
   double tbest;
   double qdst;
   d3 qn;
-  d3 ng;  // world geometry normal of the best hit (unnormalised)
+  uint32_t tri;  // the best triangle (its world normal is rebuilt after the traversal)
   uint32_t iss, prim;
   bool punched;  // the best triangle belongs to a punched surface
 };
+
+// World geometry normal (unnormalised) of mesh triangle `tri` of instance `inst`
+__device__ __forceinline__ d3 mesh_ng(const DScene& S, uint32_t inst, const ssolh_triangle& tri) {
+  const d3 v0 = ld3(tri.v0), v1 = ld3(tri.v1), v2 = ld3(tri.v2);
+  return tmul(S.insts[inst].inv_linear, cross(v2 - v0, v1 - v0));
+}
 
 // Returns true if the candidate is accepted (src/ssol_scene.c:438-536).
 __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t iss_id,
@@ -454,9 +460,7 @@ __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t is
   if (!punched) {
     if (t <= 1.e-6 || t <= r.tmin || (tri.prim == r.prim_from && iss_id == r.iss_from))
       return false;
-    const d3 v0 = ld3(tri.v0), v1 = ld3(tri.v1), v2 = ld3(tri.v2);
-    const d3 n_obj = cross(v2 - v0, v1 - v0);
-    ng_out = tmul(S.insts[is.inst].inv_linear, n_obj);
+    ng_out = mesh_ng(S, is.inst, tri);
     side = dot(ng_out, d) < 0 ? SSOL_FRONT_ : SSOL_BACK_;
   } else {
     const ssolh_inst_sshape& q = S.isshapes[iss_id];
@@ -514,7 +518,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
   ts.tbest = INFINITY;
   ts.qdst = INFINITY;
   ts.qn = mk(0, 0, 0);
-  ts.ng = mk(0, 0, 0);
+  ts.tri = 0;
   ts.iss = NONE;
   ts.prim = 0;
   ts.punched = false;
@@ -622,7 +626,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
       tmax = t < (double)FLT_MAX ? (float)t * 1.0000002f + 1e-30f : FLT_MAX;
       ts.iss = iss;
       ts.prim = tri.prim;
-      ts.ng = ng;
+      ts.tri = first + k;
       ts.punched = punched;
       if (any_hit) { done = true; break; }
     }
@@ -633,7 +637,11 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
   HitRec h;
   h.t = ts.tbest;
   h.qdst = ts.qdst;
-  const d3 n = ts.punched ? ts.qn : ts.ng;
+  d3 n = mk(0, 0, 0);
+  if (ts.punched)
+    n = ts.qn;
+  else if (ts.iss != NONE)
+    n = mesh_ng(S, S.iss[ts.iss].inst, S.tris[ts.tri]);
   h.nx = n.x; h.ny = n.y; h.nz = n.z;
   h.iss = ts.iss;
   h.prim = ts.prim;
