@@ -102,7 +102,7 @@ struct DScene {
   const ssolh_bvh_node* nodes;
   uint32_t tlas_root;
   // copies of the TLAS's top levels (breadth first, children in the set
-  // referenced as TOP_REF | index), staged in LDS by the FP64 trace kernel
+  // referenced as TOP_REF | index), staged in LDS by the trace kernels
   const ssolh_bvh_node* top;
   uint32_t ntop;
   uint32_t root_ref;         // first node reference of a traversal

@@ -169,7 +169,8 @@ template <int STACK>
 __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __restrict__ SQ,
                                             const uint32_t* __restrict__ list, uint32_t q,
                                             const RayRec32* __restrict__ rays,
-                                            HitRec32* __restrict__ hits, uint32_t* lds_stack) {
+                                            HitRec32* __restrict__ hits, uint32_t* lds_stack,
+                                            const ssolh_bvh_node* lds_top) {
 
   const uint32_t slot = slot_of(list, q);
   const RayRec32 r = rays[slot];
@@ -205,7 +206,7 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
   uint32_t iss_first = 0;
   const float ftmin = r.tmin;
 
-  uint32_t ref = S.tlas_root;
+  uint32_t ref = S.root_ref;  // the TLAS's top levels come from the LDS copy (as trace_ray)
   bool alive = true;
   auto pop = [&]() -> bool {
     if (sp == 0) return false;
@@ -216,7 +217,11 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
   float tmax = FLT_MAX;  // box culling distance (as trace_ray)
   while (alive) {
     while (!(ref & LEAF)) {
-      const ssolh_bvh_node n = S.nodes[ref];  // one round of loads, then both boxes
+      ssolh_bvh_node n;  // one round of loads, then both boxes
+      if (ref & TOP_REF)
+        n = lds_node(lds_top, ref & 0x0FFFFFFFu);
+      else
+        n = S.nodes[ref];
       float tn0, tn1;
       const bool b0 = box_hit(fr, n.lo[0], n.hi[0], ftmin, tmax, tn0);
       const bool b1 = box_hit(fr, n.lo[1], n.hi[1], ftmin, tmax, tn1);
@@ -324,15 +329,19 @@ k_trace32(DScene S, const ShapeQ* __restrict__ SQ, const uint32_t* __restrict__ 
           const unsigned* __restrict__ count_ptr, const RayRec32* __restrict__ rays,
           HitRec32* __restrict__ hits) {
   __shared__ uint32_t lds_stack[STACK * BLOCK];
+  __shared__ ssolh_bvh_node lds_top[TOP_NODES];
   const uint32_t count = *count_ptr;
+  if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: the staging syncs
+  for (uint32_t i = threadIdx.x; i < S.ntop; i += BLOCK) lds_top[i] = S.top[i];
+  __syncthreads();
   if (LOOP) {  // graph-captured launch of a fixed grid (as k_trace)
     for (uint32_t b = blockIdx.x; b * BLOCK < count; b += gridDim.x) {
       const uint32_t q = b * BLOCK + threadIdx.x;
-      if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack);
+      if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack, lds_top);
     }
   } else {
     const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-    if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack);
+    if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack, lds_top);
   }
 }
 
