@@ -61,6 +61,32 @@ def test_graph_mode_matches_host_loop(gpu):
     assert a.stats.segments == b.stats.segments
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_graph_replayed_many_times_matches_host_loop(gpu, precision):
+    """A narrow wavefront (4,096 slots refilled in place) with two
+    iterations per graph: each solve replays its captured graph dozens of
+    times, with a device-to-host copy and a synchronisation between replays,
+    and three solves in a row capture it anew; every solve equals the
+    host-driven loop's."""
+    spec = S.config2(0, n_helio=20, radius=150.0)
+    host = L.Scene(spec, precision=precision, path_slots=4096)
+    graph = L.Scene(spec, precision=precision, path_slots=4096, graph_iterations=2)
+    for seed in (1, 2, 3):
+        a, b = host.solve(200_000, seed=seed), graph.solve(200_000, seed=seed)
+        assert b.stats.iterations >= 60, b.stats.iterations  # 30+ replays
+        assert a.realisation_count == b.realisation_count == 200_000
+        assert a.stats.segments == b.stats.segments
+        ga, gb = a.global_(), b.global_()
+        for f in L.GLOBAL_FIELDS:
+            assert _same(ga[f], gb[f]), (seed, f, ga[f], gb[f])
+        rcv = spec.instances[-1]
+        ra, rb = a.receiver(rcv, S.FRONT), b.receiver(rcv, S.FRONT)
+        for f in L.RECEIVER_FIELDS:
+            assert _same(ra[f], rb[f]), (seed, f, ra[f], rb[f])
+    host.close()
+    graph.close()
+
+
 def test_batch_sweep_shape_and_physics(gpu):
     """The 512-sun sweep of the 100-paraboloid field: counts honoured, every
     sun's sinks plus shadowing balance its initial flux (DNI x sampled area x
