@@ -1158,10 +1158,14 @@ __device__ __forceinline__ void shade_block(
     const unsigned long long nf = wave_count_add(fin.kind == FIN_FAILED, &C->failed);
     if (fin.kind == FIN_FAILED && nf + 1 >= prm.max_failed) atomicExch(&C->aborted, 1u);
   }
+  // pending entries fill from entry 0 up (pend_add takes the lowest free
+  // one), so a path's keys end at its first empty entry and the wave stops
+  // at the first entry none of its finished paths uses
+  bool more = live && fin.kind != FIN_NONE;
   for (uint32_t k = 0; k < prm.npend; ++k) {
     uint32_t key = NONE;
     double v[20];
-    if (live && fin.kind != FIN_NONE) {
+    if (more) {
       key = P.pkey[k * stride + slot];
       if (key != NONE) {
         for (int f = 0; f < 10; ++f) {
@@ -1169,19 +1173,23 @@ __device__ __forceinline__ void shade_block(
           v[2 * f] = x;
           v[2 * f + 1] = x * x;
         }
+        P.pkey[k * stride + slot] = NONE;
+      } else {
+        more = false;
       }
-      P.pkey[k * stride + slot] = NONE;
     }
+    if (!__ballot(key != NONE)) break;
     const bool has = done && key != NONE;
     if (!has) for (int f = 0; f < 20; ++f) v[f] = 0;
     keyed_add<20>(has, has ? key : 0, v, rcv_base);
     const uint64_t xkey = has ? (uint64_t)fin.samp_slot * (uint64_t)S.nrcv * 2u + key : 0;
     keyed_add<20>(has, xkey, v, T.sxr + (size_t)(blockIdx.x & T.mask_x) * T.nsamp * T.nrcv * 40u);
   }
+  more = live && fin.kind != FIN_NONE;  // per-primitive entries, the same way
   for (uint32_t k = 0; k < prm.npend; ++k) {
     uint32_t key = NONE;
     double v[20];
-    if (live && fin.kind != FIN_NONE) {
+    if (more) {
       key = P.qkey[k * stride + slot];
       if (key != NONE) {
         for (int f = 0; f < 10; ++f) {
@@ -1189,9 +1197,12 @@ __device__ __forceinline__ void shade_block(
           v[2 * f] = x;
           v[2 * f + 1] = x * x;
         }
+        P.qkey[k * stride + slot] = NONE;
+      } else {
+        more = false;
       }
-      P.qkey[k * stride + slot] = NONE;
     }
+    if (!__ballot(key != NONE)) break;
     const bool has = done && key != NONE;
     if (has) {
       double* pb = T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u;
