@@ -53,7 +53,7 @@ constexpr uint32_t SENTINEL = 0xFFFFFFFFu;
 constexpr uint32_t LEAF = 0x80000000u;
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t TOP_REF = 0x40000000u;  // inner-node reference into the LDS copy
-constexpr int TOP_NODES = 31;              // TLAS nodes staged in LDS (2 KB): 5 levels
+constexpr int TOP_NODES = 63;              // TLAS nodes staged in LDS (4 KB): 6 levels
 constexpr int MAX_PEND = 8;          // compile-time bound of pending entries
 
 // Per-thread message of the last failing call
