@@ -346,6 +346,35 @@ __device__ __forceinline__ unsigned long long wave_count_add(bool pred, unsigned
   return base + (unsigned long long)__popcll(below);
 }
 
+// The logical blocks (of BLOCK list entries) of a looping, graph-captured
+// launch: the first 7/8 in static shares (block b to workgroup b mod grid),
+// the rest handed out in chunks from *fetch, which k_scan_top zeroes before
+// the next iteration. With static shares alone the kernel ends with its
+// slowest share; the handed-out tail lets the workgroups finish within about
+// one chunk of each other. A chunk is sized for about four tickets per
+// workgroup: one block per ticket made the atomics on *fetch (one address)
+// the bottleneck when the list stays full. Calls f(b) once for every block;
+// every thread of the workgroup must call it.
+template <typename F>
+__device__ __forceinline__ void for_each_block(uint32_t nb, unsigned* fetch, F&& f) {
+  __shared__ uint32_t s_b;
+  const uint32_t stat = nb - nb / 8;
+  const uint32_t chunk = max(1u, (nb - stat) / (gridDim.x * 4u));
+  uint32_t b = blockIdx.x, end = 0;  // uniform over the workgroup; [b, end): current chunk
+  for (;;) {
+    if (b >= stat && b >= end) {  // static share or chunk done: a new ticket
+      __syncthreads();  // every thread has read the previous ticket
+      if (threadIdx.x == 0) s_b = stat + atomicAdd(fetch, 1u) * chunk;
+      __syncthreads();
+      b = s_b;
+      if (b >= nb) break;
+      end = min(b + chunk, nb);
+    }
+    f(b);  // one call site: the block body is inlined once
+    b = b < stat ? b + gridDim.x : b + 1;
+  }
+}
+
 // Number of set predicates in the block, written by thread 0 to out[b]
 // (b: the logical block). Every thread of the block must call it.
 __device__ void block_count(bool pred, uint32_t* out, uint32_t b) {
@@ -654,7 +683,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
 template <int STACK, bool LOOP>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
-        const RayRec* __restrict__ rays, HitRec* __restrict__ hits) {
+        const RayRec* __restrict__ rays, HitRec* __restrict__ hits, unsigned* fetch) {
   __shared__ uint32_t lds_stack[STACK * BLOCK];
   __shared__ ssolh_bvh_node lds_top[TOP_NODES];
   const uint32_t count = *count_ptr;
@@ -662,10 +691,10 @@ k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict_
   for (uint32_t i = threadIdx.x; i < S.ntop; i += BLOCK) lds_top[i] = S.top[i];
   __syncthreads();
   if (LOOP) {
-    for (uint32_t b = blockIdx.x; b * BLOCK < count; b += gridDim.x) {
+    for_each_block((count + BLOCK - 1) / BLOCK, fetch, [&](uint32_t b) {
       const uint32_t q = b * BLOCK + threadIdx.x;
       if (q < count) trace_ray<STACK>(S, list, q, rays, hits, lds_stack, lds_top);
-    }
+    });
   } else {
     const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
     if (q < count) trace_ray<STACK>(S, list, q, rays, hits, lds_stack, lds_top);
@@ -1258,6 +1287,7 @@ struct ShadeArgs {
   Counters* C;
   AdjParams A;
   TrackParams K;
+  unsigned* fetch;  // tail-block counter of a looping launch (for_each_block)
 };
 
 // The kernel's argument segment as an Args, behind a pointer the compiler
@@ -1278,11 +1308,11 @@ k_shade(ShadeArgs a) {
   const uint32_t count = *a.count;
   // whole logical blocks only: block_count syncs (LOOP: as in k_trace)
   if (LOOP) {
-    for (uint32_t b = blockIdx.x; b * BLOCK < count; b += gridDim.x) {
+    for_each_block((count + BLOCK - 1) / BLOCK, a.fetch, [&](uint32_t b) {
       const ShadeArgs& f = kernel_args<ShadeArgs>();
       shade_block<ADJ, REFILL>(b, f.S, f.P, f.prm, f.T, f.list, count, f.rays, f.hits, f.emitted,
                                f.block_emits, f.C, f.A, f.K);
-    }
+    });
   } else if (blockIdx.x * BLOCK < count) {
     shade_block<ADJ, REFILL>(blockIdx.x, a.S, a.P, a.prm, a.T, a.list, count, a.rays, a.hits,
                              a.emitted, a.block_emits, a.C, a.A, a.K);
@@ -1341,12 +1371,15 @@ k_scan_chunks(const uint32_t* __restrict__ counts, const unsigned* __restrict__ 
 
 __global__ void __launch_bounds__(SCAN_THREADS)
 k_scan_top(const unsigned* __restrict__ count_ptr, uint32_t* __restrict__ chunk,
-           unsigned int* total, unsigned long long* segments) {
+           unsigned int* total, unsigned long long* segments, unsigned* fetch) {
   __shared__ uint32_t part[SCAN_THREADS];
   const uint32_t tid = threadIdx.x;
   const uint32_t nblocks = (*count_ptr + BLOCK - 1) / BLOCK;
   const uint32_t nchunks = (nblocks + SCAN_CHUNK - 1) / SCAN_CHUNK;
-  if (tid == 0) *segments += *count_ptr;
+  if (tid == 0) {
+    *segments += *count_ptr;
+    fetch[0] = fetch[1] = 0;  // tail-block counters of the next trace and shade
+  }
   const uint32_t v = tid < nchunks ? chunk[tid] : 0u;
   part[tid] = v;
   __syncthreads();
@@ -2026,6 +2059,9 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
 
   Counters* ctr = (Counters*)c->counters.p;
   unsigned* cnt = (unsigned*)((char*)c->counters.p + sizeof(Counters));  // [2] list lengths
+  // tail-block counters of the looping trace and shade kernels (zeroed with
+  // the counters, then by every k_scan_top)
+  unsigned* fetch = cnt + 2;
   uint32_t* lists[2] = {(uint32_t*)c->lists[0].p, (uint32_t*)c->lists[1].p};
   uint8_t* emitted = (uint8_t*)c->emitted.p;
   uint32_t* block_emits = (uint32_t*)c->block_emits.p;
@@ -2051,7 +2087,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       HitRec32* h32 = (HitRec32*)c->hits.p;
       auto launch = [&](auto kernel) {
         kernel<<<grid_of((const void*)kernel), BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list,
-                                                                     count, r32, h32);
+                                                                     count, r32, h32, fetch);
       };
       if (c->stack_depth <= 16)
         loop ? launch(k_trace32<16, true>) : launch(k_trace32<16, false>);
@@ -2066,7 +2102,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       HitRec* h64 = (HitRec*)c->hits.p;
       auto launch = [&](auto kernel) {
         kernel<<<grid_of((const void*)kernel), BLOCK, 0, c->stream>>>(c->scn, list, count, r64,
-                                                                     h64);
+                                                                     h64, fetch);
       };
       if (c->stack_depth <= 16)
         loop ? launch(k_trace<16, true>) : launch(k_trace<16, false>);
@@ -2082,7 +2118,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     if (fp32) {
       const Shade32Args args{c->scn,     c->shapeq, c->P32, prm, c->T, list, count,
                              (RayRec32*)c->rays.p, (const HitRec32*)c->hits.p, emitted,
-                             block_emits, ctr};
+                             block_emits, ctr, fetch + 1};
       auto launch = [&](auto kernel) {
         kernel<<<grid_of((const void*)kernel), BLOCK, 0, c->stream>>>(args);
       };
@@ -2090,7 +2126,8 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       else loop ? launch(k_shade32<false, true>) : launch(k_shade32<false, false>);
     } else {
       const ShadeArgs args{c->scn,  c->P,     prm,         c->T, list, count, (RayRec*)c->rays.p,
-                           (const HitRec*)c->hits.p, emitted, block_emits, ctr,  A,    K};
+                           (const HitRec*)c->hits.p, emitted, block_emits, ctr,  A,    K,
+                           fetch + 1};
       auto launch = [&](auto kernel) {
         kernel<<<grid_of((const void*)kernel), BLOCK, 0, c->stream>>>(args);
       };
@@ -2108,7 +2145,8 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     k_scan_chunks<<<(blocks + SCAN_CHUNK - 1) / SCAN_CHUNK, SCAN_THREADS, 0, c->stream>>>(
         block_emits, count, block_offsets, scan_chunks);
     HIP_TRY(hipGetLastError());
-    k_scan_top<<<1, SCAN_THREADS, 0, c->stream>>>(count, scan_chunks, cnt + (i ^ 1), &ctr->segments);
+    k_scan_top<<<1, SCAN_THREADS, 0, c->stream>>>(count, scan_chunks, cnt + (i ^ 1), &ctr->segments,
+                                                  fetch);
     HIP_TRY(hipGetLastError());
     if (loop)
       k_scatter<true><<<grid_of((const void*)k_scatter<true>), BLOCK, 0, c->stream>>>(

@@ -327,7 +327,7 @@ template <int STACK, bool LOOP>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_trace32(DScene S, const ShapeQ* __restrict__ SQ, const uint32_t* __restrict__ list,
           const unsigned* __restrict__ count_ptr, const RayRec32* __restrict__ rays,
-          HitRec32* __restrict__ hits) {
+          HitRec32* __restrict__ hits, unsigned* fetch) {
   __shared__ uint32_t lds_stack[STACK * BLOCK];
   __shared__ ssolh_bvh_node lds_top[TOP_NODES];
   const uint32_t count = *count_ptr;
@@ -335,10 +335,10 @@ k_trace32(DScene S, const ShapeQ* __restrict__ SQ, const uint32_t* __restrict__ 
   for (uint32_t i = threadIdx.x; i < S.ntop; i += BLOCK) lds_top[i] = S.top[i];
   __syncthreads();
   if (LOOP) {  // graph-captured launch of a fixed grid (as k_trace)
-    for (uint32_t b = blockIdx.x; b * BLOCK < count; b += gridDim.x) {
+    for_each_block((count + BLOCK - 1) / BLOCK, fetch, [&](uint32_t b) {
       const uint32_t q = b * BLOCK + threadIdx.x;
       if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack, lds_top);
-    }
+    });
   } else {
     const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
     if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack, lds_top);
@@ -805,6 +805,7 @@ struct Shade32Args {
   uint8_t* emitted;
   uint32_t* block_emits;
   Counters* C;
+  unsigned* fetch;  // tail-block counter of a looping launch (for_each_block)
 };
 
 template <bool REFILL, bool LOOP>
@@ -812,11 +813,11 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
 k_shade32(Shade32Args a) {
   const uint32_t count = *a.count;
   if (LOOP) {  // whole logical blocks, arguments read per block (as k_shade)
-    for (uint32_t b = blockIdx.x; b * BLOCK < count; b += gridDim.x) {
+    for_each_block((count + BLOCK - 1) / BLOCK, a.fetch, [&](uint32_t b) {
       const Shade32Args& f = kernel_args<Shade32Args>();
       shade32_block<REFILL>(b, f.S, f.SQ, f.P, f.prm, f.T, f.list, count, f.rays, f.hits,
                             f.emitted, f.block_emits, f.C);
-    }
+    });
   } else if (blockIdx.x * BLOCK < count) {
     shade32_block<REFILL>(blockIdx.x, a.S, a.SQ, a.P, a.prm, a.T, a.list, count, a.rays, a.hits,
                           a.emitted, a.block_emits, a.C);
