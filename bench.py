@@ -8,7 +8,8 @@ synthetic and seeded. A step is one ssol_solve of `--realisations` paths on
 the scene already resident in HBM (the first warm-up solve flattens and
 uploads it; later solves reuse it). The metric is Mpaths/s; the time to reach
 0.1% relative standard error on absorbed_by_receivers is measured with
-RNG-chained batches merged like src/test_ssol_solver6.c:219-235.
+RNG-chained batches (4e6 realisations, then batches sized from the merged
+standard error) merged like src/test_ssol_solver6.c:219-235.
 
 Prints one JSON line. --gpus N > 1 is refused: this build runs on one GPU.
 """
@@ -58,15 +59,21 @@ def merge(a, b):
     return E, math.sqrt(V / N), N
 
 
-def time_to_rel_se(lib, batch, target=1e-3, seed=7, max_paths=2_000_000_000):
+def time_to_rel_se(lib, batch, target=1e-3, seed=7, max_paths=2_000_000_000, max_batch=100_000_000):
+    """Wall time until absorbed_by_receivers reaches `target` relative
+    standard error: a first solve of `batch` realisations, then solves sized
+    from the merged estimate (the realisations its SE says are still needed,
+    plus 5 %), each continuing the RNG state of the previous one and merged
+    like src/test_ssol_solver6.c:219-235."""
     import ssol_ctypes as L
     t0 = time.perf_counter()
     rng = L.Rng(seed, 0)
     acc = None
     state = rng
     keep = []
+    n = batch
     while True:
-        est = lib.solve(batch, rng=state)
+        est = lib.solve(n, rng=state)
         g = est.global_()["absorbed_by_receivers"]
         cur = (g[0], g[2], est.realisation_count)
         acc = cur if acc is None else merge(acc, cur)
@@ -78,6 +85,9 @@ def time_to_rel_se(lib, batch, target=1e-3, seed=7, max_paths=2_000_000_000):
             break
         if acc[2] >= max_paths:
             break
+        rel = acc[1] / acc[0] if acc[0] > 0 else 1.0
+        need = acc[2] * (rel / target) ** 2 - acc[2]
+        n = int(min(max(1.05 * need, batch / 4), max_batch))
     return time.perf_counter() - t0, acc
 
 
