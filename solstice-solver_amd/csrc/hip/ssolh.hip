@@ -715,6 +715,16 @@ __device__ uint32_t start_medium(const DScene& S, const ssolh_material& m, uint3
   return S.air;
 }
 
+// Sample triangle of a start point from the first uniform of its stream
+// (alias table over the sampling proxy, by area)
+__device__ __forceinline__ uint32_t alias_pick(const DScene& S, double u0) {
+  const double x = u0 * (double)S.nstris;
+  uint32_t i = (uint32_t)x;
+  if (i >= S.nstris) i = S.nstris - 1;
+  const ssolh_alias al = S.alias[i];
+  return (x - (double)i) < al.prob ? i : al.alias;
+}
+
 __device__ void generate(const DScene& S, const PathState& P, const SolveParams& prm,
                          uint32_t slot, uint64_t path, Emit& e, int& samp_slot,
                          const AdjParams& A, const TrackParams& K) {
@@ -722,11 +732,7 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   rng.init(prm.key, prm.counter + path, 0);
   // area-weighted primitive through the alias table, uniform point
   const double u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
-  const double x = u0 * (double)S.nstris;
-  uint32_t i = (uint32_t)x;
-  if (i >= S.nstris) i = S.nstris - 1;
-  const ssolh_alias al = S.alias[i];
-  const uint32_t idx = (x - (double)i) < al.prob ? i : al.alias;
+  const uint32_t idx = alias_pick(S, u0);
   const ssolh_sample_tri st = S.stris[idx];
   const ssolh_instance& inst = S.insts[st.instance];
   const double sq = sqrt(u1);
@@ -825,19 +831,20 @@ __device__ void flush_sampled_count(bool has, int samp_slot, const Tallies& T) {
   if (has && !match) atomicAdd(base + key, 1ull);
 }
 
-// First realisations: slot s holds realisation s, s < count. Slot s then
-// runs realisations s + k * slots, k = 1, 2, ... while they are below N.
+// First realisations: slot s holds realisation order[s] (realisation s
+// when order is null), s < count, a permutation of [0, count). Slot s then
+// runs that realisation + k * slots, k = 1, 2, ... while below N.
 template <bool ADJ>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
 k_generate(DScene S, PathState P, SolveParams prm, Tallies T, RayRec* __restrict__ rays,
-           uint32_t count, AdjParams A_, TrackParams K) {
+           uint32_t count, AdjParams A_, TrackParams K, const uint32_t* __restrict__ order) {
   const AdjParams A = ADJ ? A_ : AdjParams{};
   const uint32_t slot = blockIdx.x * BLOCK + threadIdx.x;
   const bool live = slot < count;
   Emit e;
   e.has = false;
   int samp_slot = 0;
-  if (live) generate(S, P, prm, slot, slot, e, samp_slot, A, K);
+  if (live) generate(S, P, prm, slot, order ? order[slot] : slot, e, samp_slot, A, K);
   flush_sampled_count(live, samp_slot, T);
   if (e.has) rays[slot] = e.r;
 }
@@ -1424,6 +1431,104 @@ __global__ void k_iota(uint32_t* __restrict__ list, uint32_t n, unsigned* __rest
   if (i == 0) *count = n;
 }
 
+// Slot order of the first realisations by (sun, start instance). The
+// realisations are drawn at random over the field, so in realisation order a
+// wave's rays start on unrelated heliostats; placed by start instance, a
+// wave's rays leave neighbouring points in like directions (coherent
+// traversal) and its finished paths share tally keys (wave-reduced atomics).
+// The sun (job) comes first so a batch keeps each wave on one sun, as the
+// realisation order did. Three passes: k_start_rank draws each realisation's
+// start instance exactly as generate does and takes its rank in a histogram
+// over the keys; k_scan_small turns the histogram into offsets;
+// k_start_slots writes order[offset + rank] = q.
+template <bool FP32>
+__device__ __forceinline__ uint32_t start_key(const DScene& S, const SolveParams& prm,
+                                              uint64_t path, uint32_t ninst) {
+  uint32_t idx;
+  if (FP32) {
+    PathRng32 rng;
+    rng.init(prm.key, prm.counter + path, 0);
+    const float u0 = rng.next(), uc = rng.next();
+    idx = alias_pick32(S, u0, uc);
+  } else {
+    PathRng rng;
+    rng.init(prm.key, prm.counter + path, 0);
+    idx = alias_pick(S, rng.next());
+  }
+  return job_of(prm, path) * ninst + S.stris[idx].instance;
+}
+
+// Many keys: one global atomic per realisation (they rarely meet)
+template <bool FP32>
+__global__ void __launch_bounds__(256)
+k_start_rank(DScene S, SolveParams prm, uint32_t count, uint32_t ninst,
+             uint32_t* __restrict__ hist, uint2* __restrict__ key) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= count) return;
+  const uint32_t k = start_key<FP32>(S, prm, q, ninst);
+  key[q] = make_uint2(k, atomicAdd(hist + k, 1u));
+}
+
+// Few keys (at most START_LDS): a block ranks START_PER x 256 realisations
+// in an LDS histogram, then claims each key's range with one global atomic;
+// per-realisation atomics on a few addresses would serialise
+constexpr uint32_t START_LDS = 2048;
+constexpr uint32_t START_PER = 16;
+template <bool FP32>
+__global__ void __launch_bounds__(256)
+k_start_rank_lds(DScene S, SolveParams prm, uint32_t count, uint32_t ninst, uint32_t nkeys,
+                 uint32_t* __restrict__ hist, uint2* __restrict__ key) {
+  __shared__ uint32_t h[START_LDS];
+  for (uint32_t i = threadIdx.x; i < nkeys; i += 256) h[i] = 0;
+  __syncthreads();
+  const uint32_t q0 = blockIdx.x * 256 * START_PER + threadIdx.x;
+  uint32_t inst[START_PER], loc[START_PER];
+#pragma unroll
+  for (uint32_t k = 0; k < START_PER; ++k) {
+    const uint32_t q = q0 + k * 256;
+    inst[k] = q < count ? start_key<FP32>(S, prm, q, ninst) : 0u;
+    loc[k] = q < count ? atomicAdd(h + inst[k], 1u) : 0u;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nkeys; i += 256) {
+    const uint32_t n = h[i];
+    h[i] = n ? atomicAdd(hist + i, n) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < START_PER; ++k) {
+    const uint32_t q = q0 + k * 256;
+    if (q < count) key[q] = make_uint2(inst[k], h[inst[k]] + loc[k]);
+  }
+}
+
+// Exclusive scan of a[0, n) in place by one block (a segment per thread)
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_small(uint32_t* __restrict__ a, uint32_t n) {
+  __shared__ uint32_t part[SCAN_THREADS];
+  const uint32_t per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
+  const uint32_t lo = min(n, threadIdx.x * per), hi = min(n, lo + per);
+  uint32_t s = 0;
+  for (uint32_t i = lo; i < hi; ++i) s += a[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  block_inclusive_scan(part);
+  uint32_t run = part[threadIdx.x] - s;
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint32_t v = a[i];
+    a[i] = run;
+    run += v;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_start_slots(const uint2* __restrict__ key, const uint32_t* __restrict__ offset, uint32_t count,
+              uint32_t* __restrict__ order) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= count) return;
+  const uint2 k = key[q];
+  order[offset[k.x] + k.y] = q;
+}
+
 // Sum of the stripes of a striped tally into stripe 0
 template <typename T>
 __global__ void k_fold(T* base, size_t n, uint32_t stripes) {
@@ -1527,6 +1632,7 @@ struct ssolh_context {
   uint64_t nprim = 0;
   uint32_t stack_depth = MAX_STACK;
   // solve buffers
+  DevBuf start_hist;  // per-instance histogram / offsets of the first realisations
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, scan_chunks, counters,
       tallies, jobs;
   DevBuf adj_state, adj_in, adj_grad;
@@ -1701,7 +1807,7 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
   free_scene(c);
   for (DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
                     &c->block_emits, &c->block_offsets, &c->scan_chunks, &c->counters, &c->tallies,
-                    &c->jobs,
+                    &c->jobs, &c->start_hist,
                     &c->adj_state, &c->adj_in, &c->adj_grad, &c->trk_vtx, &c->trk_type,
                     &c->trk_seq, &c->trk_count})
     if (b->p) (void)hipFree(b->p);
@@ -2164,16 +2270,51 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   uint32_t count = (uint32_t)std::min<uint64_t>(N, slots);
   k_iota<<<(count + 255) / 256, 256, 0, c->stream>>>(lists[0], count, cnt);
   HIP_TRY(hipGetLastError());
+  // slots in start-instance order (k_start_rank): the rank and instance of
+  // each first realisation go to the hit records and the order to the
+  // second work list, neither of which is used before the first trace
+  const uint32_t* order = nullptr;
+  const uint64_t nkeys = (uint64_t)njobs * c->ninstances;
+  // Only when every realisation is in flight: with fewer slots, refilled
+  // slots take realisations r + slots, which in id order stay on one sun
+  // per wave and in this order do not (512-sun sweep 1,483 -> 1,345
+  // Mpaths/s). One sun and one sampled instance: nothing to order.
+  if (count && count == N && (uint64_t)njobs * c->nsamp > 1 && nkeys <= (1u << 24)) {
+    const uint32_t nk = (uint32_t)nkeys, ni = c->ninstances;
+    if ((r = dev_alloc(c->start_hist, (size_t)nk * sizeof(uint32_t)))) return r;
+    uint32_t* hist = (uint32_t*)c->start_hist.p;
+    uint2* key = (uint2*)c->hits.p;
+    HIP_TRY(hipMemsetAsync(hist, 0, (size_t)nk * sizeof(uint32_t), c->stream));
+    const uint32_t per_block = 256 * START_PER;
+    if (nk <= START_LDS && fp32)
+      k_start_rank_lds<true><<<(count + per_block - 1) / per_block, 256, 0, c->stream>>>(
+          c->scn, prm, count, ni, nk, hist, key);
+    else if (nk <= START_LDS)
+      k_start_rank_lds<false><<<(count + per_block - 1) / per_block, 256, 0, c->stream>>>(
+          c->scn, prm, count, ni, nk, hist, key);
+    else if (fp32)
+      k_start_rank<true><<<(count + 255) / 256, 256, 0, c->stream>>>(c->scn, prm, count, ni, hist,
+                                                                     key);
+    else
+      k_start_rank<false><<<(count + 255) / 256, 256, 0, c->stream>>>(c->scn, prm, count, ni, hist,
+                                                                      key);
+    HIP_TRY(hipGetLastError());
+    k_scan_small<<<1, SCAN_THREADS, 0, c->stream>>>(hist, nk);
+    HIP_TRY(hipGetLastError());
+    k_start_slots<<<(count + 255) / 256, 256, 0, c->stream>>>(key, hist, count, lists[1]);
+    HIP_TRY(hipGetLastError());
+    order = lists[1];
+  }
   if (fp32)
     k_generate32<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
-        c->scn, c->shapeq, c->P32, prm, c->T, (RayRec32*)c->rays.p, count);
+        c->scn, c->shapeq, c->P32, prm, c->T, (RayRec32*)c->rays.p, count, order);
   else
     if (A.grad)
       k_generate<true><<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
-          c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A, K);
+          c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A, K, order);
     else
       k_generate<false><<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
-          c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A, K);
+          c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A, K, order);
   HIP_TRY(hipGetLastError());
   uint32_t iter = 0;
   if (!graph_iters) {

@@ -351,16 +351,22 @@ struct Emit32 {
   RayRec32 r;
 };
 
+// Sample triangle of a start point from the first two uniforms of its
+// stream (FP32 form of alias_pick)
+__device__ __forceinline__ uint32_t alias_pick32(const DScene& S, float u0, float uc) {
+  uint32_t i = (uint32_t)(u0 * (float)S.nstris);
+  if (i >= S.nstris) i = S.nstris - 1;
+  const ssolh_alias al = S.alias[i];
+  return (double)uc < al.prob ? i : al.alias;
+}
+
 __device__ void generate32(const DScene& S, const ShapeQ* SQ, const PathState32& P,
                            const SolveParams& prm, uint32_t slot, uint64_t path, Emit32& e,
                            int& samp_slot) {
   PathRng32 rng;
   rng.init(prm.key, prm.counter + path, 0);
   const float u0 = rng.next(), uc = rng.next(), u1 = rng.next(), u2 = rng.next();
-  uint32_t i = (uint32_t)(u0 * (float)S.nstris);
-  if (i >= S.nstris) i = S.nstris - 1;
-  const ssolh_alias al = S.alias[i];
-  const uint32_t idx = (double)uc < al.prob ? i : al.alias;
+  const uint32_t idx = alias_pick32(S, u0, uc);
   const ssolh_sample_tri st = S.stris[idx];
   const ssolh_instance& inst = S.insts[st.instance];
   const float sq = sqrtf(u1);
@@ -431,13 +437,13 @@ __device__ void generate32(const DScene& S, const ShapeQ* SQ, const PathState32&
 
 __global__ void __launch_bounds__(BLOCK)
 k_generate32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams prm, Tallies T,
-             RayRec32* __restrict__ rays, uint32_t count) {
+             RayRec32* __restrict__ rays, uint32_t count, const uint32_t* __restrict__ order) {
   const uint32_t slot = blockIdx.x * BLOCK + threadIdx.x;
   const bool live = slot < count;
   Emit32 e;
   e.has = false;
   int samp_slot = 0;
-  if (live) generate32(S, SQ, P, prm, slot, slot, e, samp_slot);
+  if (live) generate32(S, SQ, P, prm, slot, order ? order[slot] : slot, e, samp_slot);  // as k_generate
   flush_sampled_count(live, samp_slot, T);
   if (e.has) rays[slot] = e.r;
 }
