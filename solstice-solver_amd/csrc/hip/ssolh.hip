@@ -1246,16 +1246,20 @@ __device__ __forceinline__ void shade_block(
     }
   }
 
-  // --- adjoint: the finished path's weighted contribution times its scores
-  if (A.grad && done) {
-    const uint32_t pp = A.p[slot];
-    if (pp != NONE) {
+  // --- adjoint: the finished path's weighted contribution times its scores,
+  // wave-reduced (in start-instance order a wave's paths share a heliostat)
+  if (A.grad) {
+    const uint32_t pp = done ? A.p[slot] : NONE;
+    const bool has = pp != NONE;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (has) {
       const double4 sc = A.s[slot];
       const double q = sc.w * fin.factor;
-      atomic_add(A.grad + 3 * (size_t)pp, q * sc.x);
-      atomic_add(A.grad + 3 * (size_t)pp + 1, q * sc.y);
-      atomic_add(A.grad + 3 * (size_t)pp + 2, q * sc.z);
+      v[0] = q * sc.x;
+      v[1] = q * sc.y;
+      v[2] = q * sc.z;
     }
+    keyed_add<3>(has, has ? pp : 0u, v, A.grad);
   }
 
   // --- refill finished slots with their next realisation -------------------
