@@ -1473,25 +1473,26 @@ k_start_rank(DScene S, SolveParams prm, uint32_t count, uint32_t ninst,
   key[q] = make_uint2(k, atomicAdd(hist + k, 1u));
 }
 
-// Few keys (at most START_LDS): a block ranks START_PER x 256 realisations
-// in an LDS histogram, then claims each key's range with one global atomic;
-// per-realisation atomics on a few addresses would serialise
-constexpr uint32_t START_LDS = 2048;
-constexpr uint32_t START_PER = 16;
+// Up to START_LDS keys: a block ranks per_block realisations (up to
+// START_PER_BLOCK; the host aims at about 1,024 blocks) in an LDS histogram,
+// then claims each key's range with one global atomic and adds that base to
+// its realisations' ranks. With many realisations per block a key's global
+// atomics drop from one per realisation to one per block (one heliostat, or
+// a few: otherwise all on one address or a few).
+constexpr uint32_t START_LDS = 16384;          // 64 KB of LDS
+constexpr uint32_t START_PER_BLOCK = 65536;
 template <bool FP32>
 __global__ void __launch_bounds__(256)
 k_start_rank_lds(DScene S, SolveParams prm, uint32_t count, uint32_t ninst, uint32_t nkeys,
-                 uint32_t* __restrict__ hist, uint2* __restrict__ key) {
+                 uint32_t per_block, uint32_t* __restrict__ hist, uint2* __restrict__ key) {
   __shared__ uint32_t h[START_LDS];
   for (uint32_t i = threadIdx.x; i < nkeys; i += 256) h[i] = 0;
   __syncthreads();
-  const uint32_t q0 = blockIdx.x * 256 * START_PER + threadIdx.x;
-  uint32_t inst[START_PER], loc[START_PER];
-#pragma unroll
-  for (uint32_t k = 0; k < START_PER; ++k) {
-    const uint32_t q = q0 + k * 256;
-    inst[k] = q < count ? start_key<FP32>(S, prm, q, ninst) : 0u;
-    loc[k] = q < count ? atomicAdd(h + inst[k], 1u) : 0u;
+  const uint32_t q0 = blockIdx.x * per_block;
+  const uint32_t q1 = min(count, q0 + per_block);
+  for (uint32_t q = q0 + threadIdx.x; q < q1; q += 256) {
+    const uint32_t k = start_key<FP32>(S, prm, q, ninst);
+    key[q] = make_uint2(k, atomicAdd(h + k, 1u));
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nkeys; i += 256) {
@@ -1499,10 +1500,10 @@ k_start_rank_lds(DScene S, SolveParams prm, uint32_t count, uint32_t ninst, uint
     h[i] = n ? atomicAdd(hist + i, n) : 0u;
   }
   __syncthreads();
-#pragma unroll
-  for (uint32_t k = 0; k < START_PER; ++k) {
-    const uint32_t q = q0 + k * 256;
-    if (q < count) key[q] = make_uint2(inst[k], h[inst[k]] + loc[k]);
+  for (uint32_t q = q0 + threadIdx.x; q < q1; q += 256) {
+    uint2 v = key[q];
+    v.y += h[v.x];
+    key[q] = v;
   }
 }
 
@@ -2289,13 +2290,14 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     uint32_t* hist = (uint32_t*)c->start_hist.p;
     uint2* key = (uint2*)c->hits.p;
     HIP_TRY(hipMemsetAsync(hist, 0, (size_t)nk * sizeof(uint32_t), c->stream));
-    const uint32_t per_block = 256 * START_PER;
+    const uint32_t per_block =
+        std::min<uint32_t>(START_PER_BLOCK, std::max<uint32_t>(4096, (count / 1024 + 255) / 256 * 256));
     if (nk <= START_LDS && fp32)
       k_start_rank_lds<true><<<(count + per_block - 1) / per_block, 256, 0, c->stream>>>(
-          c->scn, prm, count, ni, nk, hist, key);
+          c->scn, prm, count, ni, nk, per_block, hist, key);
     else if (nk <= START_LDS)
       k_start_rank_lds<false><<<(count + per_block - 1) / per_block, 256, 0, c->stream>>>(
-          c->scn, prm, count, ni, nk, hist, key);
+          c->scn, prm, count, ni, nk, per_block, hist, key);
     else if (fp32)
       k_start_rank<true><<<(count + 255) / 256, 256, 0, c->stream>>>(c->scn, prm, count, ni, hist,
                                                                      key);
