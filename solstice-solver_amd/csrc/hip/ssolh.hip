@@ -831,20 +831,22 @@ __device__ void flush_sampled_count(bool has, int samp_slot, const Tallies& T) {
   if (has && !match) atomicAdd(base + key, 1ull);
 }
 
-// First realisations: slot s holds realisation order[s] (realisation s
-// when order is null), s < count, a permutation of [0, count). Slot s then
-// runs that realisation + k * slots, k = 1, 2, ... while below N.
+// First realisations of a wavefront: slot s holds realisation first +
+// order[s] (first + s when order is null), s < count, order a permutation
+// of [0, count). With refill, slot s then runs that realisation + k * slots,
+// k = 1, 2, ... while below N.
 template <bool ADJ>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
 k_generate(DScene S, PathState P, SolveParams prm, Tallies T, RayRec* __restrict__ rays,
-           uint32_t count, AdjParams A_, TrackParams K, const uint32_t* __restrict__ order) {
+           uint64_t first, uint32_t count, AdjParams A_, TrackParams K,
+           const uint32_t* __restrict__ order) {
   const AdjParams A = ADJ ? A_ : AdjParams{};
   const uint32_t slot = blockIdx.x * BLOCK + threadIdx.x;
   const bool live = slot < count;
   Emit e;
   e.has = false;
   int samp_slot = 0;
-  if (live) generate(S, P, prm, slot, order ? order[slot] : slot, e, samp_slot, A, K);
+  if (live) generate(S, P, prm, slot, first + (order ? order[slot] : slot), e, samp_slot, A, K);
   flush_sampled_count(live, samp_slot, T);
   if (e.has) rays[slot] = e.r;
 }
@@ -1465,11 +1467,11 @@ __device__ __forceinline__ uint32_t start_key(const DScene& S, const SolveParams
 // Many keys: one global atomic per realisation (they rarely meet)
 template <bool FP32>
 __global__ void __launch_bounds__(256)
-k_start_rank(DScene S, SolveParams prm, uint32_t count, uint32_t ninst,
+k_start_rank(DScene S, SolveParams prm, uint64_t first, uint32_t count, uint32_t ninst,
              uint32_t* __restrict__ hist, uint2* __restrict__ key) {
   const uint32_t q = blockIdx.x * 256 + threadIdx.x;
   if (q >= count) return;
-  const uint32_t k = start_key<FP32>(S, prm, q, ninst);
+  const uint32_t k = start_key<FP32>(S, prm, first + q, ninst);
   key[q] = make_uint2(k, atomicAdd(hist + k, 1u));
 }
 
@@ -1483,15 +1485,16 @@ constexpr uint32_t START_LDS = 16384;          // 64 KB of LDS
 constexpr uint32_t START_PER_BLOCK = 65536;
 template <bool FP32>
 __global__ void __launch_bounds__(256)
-k_start_rank_lds(DScene S, SolveParams prm, uint32_t count, uint32_t ninst, uint32_t nkeys,
-                 uint32_t per_block, uint32_t* __restrict__ hist, uint2* __restrict__ key) {
+k_start_rank_lds(DScene S, SolveParams prm, uint64_t first, uint32_t count, uint32_t ninst,
+                 uint32_t nkeys, uint32_t per_block, uint32_t* __restrict__ hist,
+                 uint2* __restrict__ key) {
   __shared__ uint32_t h[START_LDS];
   for (uint32_t i = threadIdx.x; i < nkeys; i += 256) h[i] = 0;
   __syncthreads();
   const uint32_t q0 = blockIdx.x * per_block;
   const uint32_t q1 = min(count, q0 + per_block);
   for (uint32_t q = q0 + threadIdx.x; q < q1; q += 256) {
-    const uint32_t k = start_key<FP32>(S, prm, q, ninst);
+    const uint32_t k = start_key<FP32>(S, prm, first + q, ninst);
     key[q] = make_uint2(k, atomicAdd(h + k, 1u));
   }
   __syncthreads();
@@ -2179,6 +2182,17 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   uint32_t* block_offsets = (uint32_t*)c->block_offsets.p;
   uint32_t* scan_chunks = (uint32_t*)c->scan_chunks.p;
 
+  // With fewer slots than realisations and something to order by (more than
+  // one sun or sampled instance) that ranks in LDS (at most START_LDS keys),
+  // the solve runs in chunks of `slots` realisations, each all in flight and
+  // in start-instance order, instead of refilling slots in place
+  // (100-paraboloid field at 2^20 slots: 449 -> 629 Mpaths/s). With more
+  // keys the ranking of every chunk costs more than the order gains (512-sun
+  // sweep, 51,712 keys: 1,483 -> 1,200), and slots are refilled.
+  const uint64_t nkeys = (uint64_t)njobs * c->ninstances;
+  const bool orderable = (uint64_t)njobs * c->nsamp > 1 && nkeys <= (1u << 24);
+  const bool chunked = slots < N && orderable && nkeys <= START_LDS;
+
   // one iteration: trace, shade, compaction of list i into list i^1
   auto iteration = [&](int i, uint32_t blocks) -> int {
     // blocks: logical blocks of the list (of its largest length under graph
@@ -2192,7 +2206,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     };
     const uint32_t* list = lists[i];
     const unsigned* count = cnt + i;
-    const bool refill = slots < N;
+    const bool refill = slots < N && !chunked;
     if (fp32) {
       RayRec32* r32 = (RayRec32*)c->rays.p;
       HitRec32* h32 = (HitRec32*)c->hits.p;
@@ -2271,59 +2285,61 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
 
   float ms_trace = 0, ms_shade = 0;
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  // the first realisations fill slots [0, count): list 0 is 0..count-1
-  uint32_t count = (uint32_t)std::min<uint64_t>(N, slots);
-  k_iota<<<(count + 255) / 256, 256, 0, c->stream>>>(lists[0], count, cnt);
-  HIP_TRY(hipGetLastError());
-  // slots in start-instance order (k_start_rank): the rank and instance of
-  // each first realisation go to the hit records and the order to the
-  // second work list, neither of which is used before the first trace
-  const uint32_t* order = nullptr;
-  const uint64_t nkeys = (uint64_t)njobs * c->ninstances;
-  // Only when every realisation is in flight: with fewer slots, refilled
-  // slots take realisations r + slots, which in id order stay on one sun
-  // per wave and in this order do not (512-sun sweep 1,483 -> 1,345
-  // Mpaths/s). One sun and one sampled instance: nothing to order.
-  if (count && count == N && (uint64_t)njobs * c->nsamp > 1 && nkeys <= (1u << 24)) {
-    const uint32_t nk = (uint32_t)nkeys, ni = c->ninstances;
-    if ((r = dev_alloc(c->start_hist, (size_t)nk * sizeof(uint32_t)))) return r;
-    uint32_t* hist = (uint32_t*)c->start_hist.p;
-    uint2* key = (uint2*)c->hits.p;
-    HIP_TRY(hipMemsetAsync(hist, 0, (size_t)nk * sizeof(uint32_t), c->stream));
-    const uint32_t per_block =
-        std::min<uint32_t>(START_PER_BLOCK, std::max<uint32_t>(4096, (count / 1024 + 255) / 256 * 256));
-    if (nk <= START_LDS && fp32)
-      k_start_rank_lds<true><<<(count + per_block - 1) / per_block, 256, 0, c->stream>>>(
-          c->scn, prm, count, ni, nk, per_block, hist, key);
-    else if (nk <= START_LDS)
-      k_start_rank_lds<false><<<(count + per_block - 1) / per_block, 256, 0, c->stream>>>(
-          c->scn, prm, count, ni, nk, per_block, hist, key);
-    else if (fp32)
-      k_start_rank<true><<<(count + 255) / 256, 256, 0, c->stream>>>(c->scn, prm, count, ni, hist,
-                                                                     key);
+
+  // The first realisations of a wavefront, first .. first + count - 1, fill
+  // slots [0, count) (list 0 is 0..count-1): in start-instance order when
+  // no slot is refilled (a chunk, or every realisation in flight; refilled
+  // slots take realisations r + slots, which in id order stay on one sun per
+  // wave and in this order would not). k_start_rank puts the rank and
+  // instance of each in the hit records and the order in the second work
+  // list, neither of which is used before the first trace.
+  auto start = [&](uint64_t first, uint32_t count) -> int {
+    k_iota<<<(count + 255) / 256, 256, 0, c->stream>>>(lists[0], count, cnt);
+    HIP_TRY(hipGetLastError());
+    const uint32_t* order = nullptr;
+    if (count && (chunked || first + count == N) && orderable) {
+      const uint32_t nk = (uint32_t)nkeys, ni = c->ninstances;
+      if (int e = dev_alloc(c->start_hist, (size_t)nk * sizeof(uint32_t))) return e;
+      uint32_t* hist = (uint32_t*)c->start_hist.p;
+      uint2* key = (uint2*)c->hits.p;
+      HIP_TRY(hipMemsetAsync(hist, 0, (size_t)nk * sizeof(uint32_t), c->stream));
+      const uint32_t per_block = std::min<uint32_t>(
+          START_PER_BLOCK, std::max<uint32_t>(4096, (count / 1024 + 255) / 256 * 256));
+      const uint32_t gl = (count + per_block - 1) / per_block, gg = (count + 255) / 256;
+      if (nk <= START_LDS && fp32)
+        k_start_rank_lds<true><<<gl, 256, 0, c->stream>>>(c->scn, prm, first, count, ni, nk,
+                                                          per_block, hist, key);
+      else if (nk <= START_LDS)
+        k_start_rank_lds<false><<<gl, 256, 0, c->stream>>>(c->scn, prm, first, count, ni, nk,
+                                                           per_block, hist, key);
+      else if (fp32)
+        k_start_rank<true><<<gg, 256, 0, c->stream>>>(c->scn, prm, first, count, ni, hist, key);
+      else
+        k_start_rank<false><<<gg, 256, 0, c->stream>>>(c->scn, prm, first, count, ni, hist, key);
+      HIP_TRY(hipGetLastError());
+      k_scan_small<<<1, SCAN_THREADS, 0, c->stream>>>(hist, nk);
+      HIP_TRY(hipGetLastError());
+      k_start_slots<<<gg, 256, 0, c->stream>>>(key, hist, count, lists[1]);
+      HIP_TRY(hipGetLastError());
+      order = lists[1];
+    }
+    const uint32_t gb = (count + BLOCK - 1) / BLOCK;
+    if (fp32)
+      k_generate32<<<gb, BLOCK, 0, c->stream>>>(c->scn, c->shapeq, c->P32, prm, c->T,
+                                                (RayRec32*)c->rays.p, first, count, order);
+    else if (A.grad)
+      k_generate<true><<<gb, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, (RayRec*)c->rays.p,
+                                                    first, count, A, K, order);
     else
-      k_start_rank<false><<<(count + 255) / 256, 256, 0, c->stream>>>(c->scn, prm, count, ni, hist,
-                                                                      key);
+      k_generate<false><<<gb, BLOCK, 0, c->stream>>>(c->scn, c->P, prm, c->T, (RayRec*)c->rays.p,
+                                                     first, count, A, K, order);
     HIP_TRY(hipGetLastError());
-    k_scan_small<<<1, SCAN_THREADS, 0, c->stream>>>(hist, nk);
-    HIP_TRY(hipGetLastError());
-    k_start_slots<<<(count + 255) / 256, 256, 0, c->stream>>>(key, hist, count, lists[1]);
-    HIP_TRY(hipGetLastError());
-    order = lists[1];
-  }
-  if (fp32)
-    k_generate32<<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
-        c->scn, c->shapeq, c->P32, prm, c->T, (RayRec32*)c->rays.p, count, order);
-  else
-    if (A.grad)
-      k_generate<true><<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
-          c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A, K, order);
-    else
-      k_generate<false><<<(count + BLOCK - 1) / BLOCK, BLOCK, 0, c->stream>>>(
-          c->scn, c->P, prm, c->T, (RayRec*)c->rays.p, count, A, K, order);
-  HIP_TRY(hipGetLastError());
+    return 0;
+  };
+
   uint32_t iter = 0;
-  if (!graph_iters) {
+  // host-driven iterations until the list is empty
+  auto drain_host = [&](uint32_t count) -> int {
     int cur = 0;
     while (count) {
       if (iter >= 1000000u) {
@@ -2331,7 +2347,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
         return SSOLH_ERR_LIMIT;
       }
       HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-      if ((r = iteration(cur, (count + BLOCK - 1) / BLOCK))) return r;
+      if (int e = iteration(cur, (count + BLOCK - 1) / BLOCK)) return e;
       unsigned total = 0;
       HIP_TRY(hipMemcpyAsync(&total, cnt + (cur ^ 1), sizeof(total), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
@@ -2348,44 +2364,62 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       count = total;
       ++iter;
     }
-  } else {
-    // graph of an even number of iterations: list 0 -> 1 -> 0 ...; every
-    // kernel reads its work count from device memory, so iterations past
-    // the end of the work are empty launches
-    const uint32_t gi = (graph_iters + 1) & ~1u;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int cr = 0;
-    for (uint32_t k = 0; k < gi && !cr; ++k) cr = iteration((int)(k & 1), max_blocks);
-    const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
-    if (cr) { if (graph) (void)hipGraphDestroy(graph); return cr; }
-    HIP_TRY(ce);
-    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      HIP_TRY(ie);
-    }
-    int gerr = 0;
+    return 0;
+  };
+
+  // graph of an even number of iterations: list 0 -> 1 -> 0 ...; every
+  // kernel reads its work count from device memory, so iterations past the
+  // end of the work are empty launches. Captured once, replayed for every
+  // chunk until its list is empty.
+  const uint32_t gi = (graph_iters + 1) & ~1u;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  auto drain_graph = [&](uint32_t count) -> int {
     while (count) {
       if (iter >= 1000000u) {
         set_err("ssolh_solve: the wavefront did not drain after %u iterations", iter);
-        gerr = SSOLH_ERR_LIMIT;
-        break;
+        return SSOLH_ERR_LIMIT;
       }
       const hipError_t le = hipGraphLaunch(exec, c->stream);
-      if (le != hipSuccess) { set_err("hipGraphLaunch: %s", hipGetErrorString(le)); gerr = -(int)le; break; }
+      if (le != hipSuccess) { set_err("hipGraphLaunch: %s", hipGetErrorString(le)); return -(int)le; }
       unsigned total = 0;
       const hipError_t me = hipMemcpyAsync(&total, cnt, sizeof(total), hipMemcpyDeviceToHost, c->stream);
       const hipError_t se = me == hipSuccess ? hipStreamSynchronize(c->stream) : me;
-      if (se != hipSuccess) { set_err("graph solve: %s", hipGetErrorString(se)); gerr = -(int)se; break; }
+      if (se != hipSuccess) { set_err("graph solve: %s", hipGetErrorString(se)); return -(int)se; }
       count = total;
       iter += gi;
     }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    if (gerr) return gerr;
+    return 0;
+  };
+
+  const uint64_t chunk = chunked ? slots : N;  // realisations started per wavefront
+  int gerr = 0;
+  for (uint64_t first = 0; first < N && !gerr; first += chunk) {
+    const uint32_t count = (uint32_t)std::min<uint64_t>(N - first, slots);
+    if ((gerr = start(first, count))) break;
+    if (!graph_iters) {
+      gerr = drain_host(count);
+    } else {
+      if (!exec) {
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int cr = 0;
+        for (uint32_t k = 0; k < gi && !cr; ++k) cr = iteration((int)(k & 1), max_blocks);
+        const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+        if (cr) { if (graph) (void)hipGraphDestroy(graph); return cr; }
+        HIP_TRY(ce);
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+          (void)hipGraphDestroy(graph);
+          HIP_TRY(ie);
+        }
+      }
+      gerr = drain_graph(count);
+    }
+    if (!chunked) break;  // refill: one wavefront runs every realisation
   }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (gerr) return gerr;
   // fold the stripes
   auto fold = [&](auto* base, size_t nn, uint32_t stripes) -> int {
     if (stripes > 1 && nn) {

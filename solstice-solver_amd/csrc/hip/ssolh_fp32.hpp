@@ -437,13 +437,14 @@ __device__ void generate32(const DScene& S, const ShapeQ* SQ, const PathState32&
 
 __global__ void __launch_bounds__(BLOCK)
 k_generate32(DScene S, const ShapeQ* __restrict__ SQ, PathState32 P, SolveParams prm, Tallies T,
-             RayRec32* __restrict__ rays, uint32_t count, const uint32_t* __restrict__ order) {
+             RayRec32* __restrict__ rays, uint64_t first, uint32_t count,
+             const uint32_t* __restrict__ order) {
   const uint32_t slot = blockIdx.x * BLOCK + threadIdx.x;
   const bool live = slot < count;
   Emit32 e;
   e.has = false;
   int samp_slot = 0;
-  if (live) generate32(S, SQ, P, prm, slot, order ? order[slot] : slot, e, samp_slot);  // as k_generate
+  if (live) generate32(S, SQ, P, prm, slot, first + (order ? order[slot] : slot), e, samp_slot);  // as k_generate
   flush_sampled_count(live, samp_slot, T);
   if (e.has) rays[slot] = e.r;
 }
