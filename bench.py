@@ -288,9 +288,9 @@ def main(argv=None):
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs the flux integrator on one GPU; --gpus {args.gpus} "
-              "is not supported (the realisations shard with no exchange but a tally "
-              "all-reduce; see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py: the flux integrator runs on one GPU; --gpus {args.gpus} is not "
+              "supported (across GPUs the realisations would shard by id with one "
+              "all-reduce of the folded tallies).", file=sys.stderr)
         return 2
 
     import ssol_scene as S
