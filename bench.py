@@ -249,6 +249,32 @@ def adjoint_timing(n=20_000_000):
     return out
 
 
+def host_cores():
+    """Cores this process may use: the CPU affinity set, bounded by the
+    cgroup CPU quota when one is set (a container's share of the host)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) / int(period)))
+    except (OSError, ValueError):
+        pass
+    return (min(n, quota) if quota else n), n, quota
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(spec, seconds: float, threads: int):
     """The oracle (FP64 OpenMP restatement of the reference solver) on the
     host cores, on a bounded sample of the same workload."""
@@ -265,7 +291,32 @@ def cpu_baseline(spec, seconds: float, threads: int):
         n += chunk
         k += 1
     dt = time.perf_counter() - t0
-    return n / dt / 1e6, n, dt
+    return n / dt / 1e6, n, dt, o
+
+
+def cpu_time_to_rel_se(o, batch, threads, target=1e-3, seed=7, max_seconds=60.0):
+    """The progressive recipe of time_to_rel_se on the oracle: a first solve
+    of `batch` realisations, then solves sized from the merged estimate,
+    each continuing the RNG counter of the previous one, merged like
+    src/test_ssol_solver6.c:219-235. Stops at the target or after
+    max_seconds (then returns the time so far and the estimate reached)."""
+    t0 = time.perf_counter()
+    acc = None
+    counter = 0
+    n = batch
+    while True:
+        o.solve(n, seed=seed, counter=counter, nthreads=threads)
+        counter += n
+        g = o.global_()["absorbed_by_receivers"]
+        cur = (g[0], g[2], o.realisation_count)
+        acc = cur if acc is None else merge(acc, cur)
+        if acc[0] > 0 and acc[1] / acc[0] <= target:
+            return time.perf_counter() - t0, acc, True
+        if time.perf_counter() - t0 > max_seconds:
+            return time.perf_counter() - t0, acc, False
+        rel = acc[1] / acc[0] if acc[0] > 0 else 1.0
+        need = acc[2] * (rel / target) ** 2 - acc[2]
+        n = int(max(1.05 * need, batch / 4))
 
 
 def main(argv=None):
@@ -280,6 +331,8 @@ def main(argv=None):
     ap.add_argument("--graph", type=int, default=0,
                     help="wavefront iterations per HIP graph launch (0: host-driven)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-ttt-seconds", type=float, default=40.0,
+                    help="bound on the CPU time-to-0.1%% run (0: skip it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth yardstick")
     ap.add_argument("--no-ttt", action="store_true", help="skip the time-to-0.1%% run")
@@ -305,9 +358,14 @@ def main(argv=None):
                   graph_iterations=args.graph)
     B = args.realisations
 
+    # the first solve is cold: it flattens the scene (BVHs, sampling tables),
+    # creates the GPU context, uploads the scene and allocates the wavefront
+    cold_s = None
     for w in range(args.warmup):
         t = time.perf_counter()
         est = lib.solve(B, seed=100 + w)
+        if w == 0:
+            cold_s = time.perf_counter() - t
         print(f"[bench] warmup {w}: {time.perf_counter() - t:.3f} s", file=sys.stderr, flush=True)
 
     times, segs, trace_ms, shade_ms, iters = [], 0, 0.0, 0.0, 0
@@ -371,6 +429,8 @@ def main(argv=None):
         "roofline": roofline,
         "absorbed_by_receivers": {"E": g["absorbed_by_receivers"][0],
                                   "SE": g["absorbed_by_receivers"][2]},
+        "cold_first_solve_s": None if cold_s is None else round(cold_s, 4),
+        "warm_solve_s": round(total / args.steps, 4),
     }
 
     if not args.no_ttt:
@@ -392,18 +452,24 @@ def main(argv=None):
         out["adjoint_1000_heliostats"] = adjoint_timing()
 
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        mps, n, dt = cpu_baseline(spec, args.cpu_seconds, threads)
+        threads, affinity, quota = host_cores()
+        mps, n, dt, o = cpu_baseline(spec, args.cpu_seconds, threads)
         out["cpu_baseline"] = {"value": round(mps, 4), "unit": "Mpaths/s", "cores": threads,
                                "kind": "port",
                                "sample": f"{n} paths of the same workload in {dt:.1f} s "
                                          "(oracle: FP64 OpenMP restatement of the reference "
-                                         "solver with its own SAH BVH)"}
-        if "time_to_0.1pct_rel_se_s" in out:
-            out["cpu_baseline"]["time_to_0.1pct_rel_se_s_est"] = round(
-                out["paths_to_0.1pct_rel_se"] / (mps * 1e6), 1)
-        print(f"[bench] CPU baseline: {mps:.4f} Mpaths/s on {threads} threads", file=sys.stderr,
-              flush=True)
+                                         "solver with its own SAH BVH)",
+                               "cpu_model": cpu_model(), "cpus_in_affinity_set": affinity,
+                               "cgroup_cpu_quota": quota}
+        if "time_to_0.1pct_rel_se_s" in out and args.cpu_ttt_seconds > 0:
+            t, acc, reached = cpu_time_to_rel_se(o, 1_000_000, threads,
+                                                 max_seconds=args.cpu_ttt_seconds)
+            key = "time_to_0.1pct_rel_se_s" if reached else "time_to_0.1pct_rel_se_s_lower_bound"
+            out["cpu_baseline"][key] = round(t, 2)
+            out["cpu_baseline"]["paths_to_0.1pct_rel_se"] = acc[2]
+            out["cpu_baseline"]["rel_se_reached"] = acc[1] / acc[0]
+        print(f"[bench] CPU baseline: {mps:.4f} Mpaths/s on {threads} threads "
+              f"({affinity} CPUs in the affinity set, quota {quota})", file=sys.stderr, flush=True)
 
     print(json.dumps(out), flush=True)
     return 0

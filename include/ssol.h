@@ -636,6 +636,9 @@ struct ssol_solve_stats {
   double trace_ms;
   double shade_ms;
   unsigned iterations;
+  unsigned wavefronts; /* wavefronts started: chunks of realisations all in
+                          flight (a narrow width), or 1 */
+  unsigned refilled;   /* 1: slots took further realisations in place */
 };
 SSOL_API res_T ssol_estimator_get_solve_stats
   (const struct ssol_estimator* estimator, struct ssol_solve_stats* stats);

@@ -314,6 +314,9 @@ struct ssolh_tallies {
   uint64_t pending_spills;
   uint32_t aborted;           /* failures reached max_failed */
   uint32_t iterations;
+  uint32_t wavefronts;        /* wavefronts started: chunks of realisations all
+                                 in flight, or 1 */
+  uint32_t refilled;          /* 1: slots took further realisations in place */
   double kernel_ms, trace_ms, shade_ms;
 };
 

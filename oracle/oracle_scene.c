@@ -109,37 +109,34 @@ static int emit_fan(struct dvec* xy, const p2* p, size_t n)
   return 0;
 }
 
-/* Apply one convex carving to a triangle soup (xy: 6 doubles per triangle).
- * AND keeps the part inside the polygon, SUB removes it. */
-static int carve(struct dvec* soup, const struct ocarving* c)
+/* Apply one convex polygon (n vertices, orientation ccw = +1 or -1) to a
+ * triangle soup (xy: 6 doubles per triangle); the result is appended to
+ * `out`. AND keeps the part inside the polygon, SUB removes it. */
+static int carve_convex(const struct dvec* soup, const p2* poly, size_t n,
+  int ccw, int operation, struct dvec* out)
 {
-  struct dvec out = {0};
-  p2* poly;
-  p2 buf0[64], buf1[64];
-  size_t i, k, n = c->nverts;
-  int ccw;
-  if(n < 3) { soup->n = 0; return 0; }
-  poly = malloc(n * sizeof(p2));
-  if(!poly) return -1;
-  for(i = 0; i < n; ++i) { poly[i].x = c->xy[2*i]; poly[i].y = c->xy[2*i+1]; }
-  if(!poly_is_convex(poly, n) || n > 60) { free(poly); return -2; }
-  ccw = poly_area2(poly, n) > 0 ? 1 : -1;
-  for(i = 0; i + 6 <= soup->n; i += 6) {
+  /* a triangle clipped by n half planes keeps at most n + 3 vertices */
+  p2* buf0 = malloc((n + 4) * sizeof(p2));
+  p2* buf1 = malloc((n + 4) * sizeof(p2));
+  size_t i, k;
+  int res = 0;
+  if(!buf0 || !buf1) { free(buf0); free(buf1); return -1; }
+  for(i = 0; i + 6 <= soup->n && !res; i += 6) {
     p2 tri[3];
     tri[0].x = soup->p[i]; tri[0].y = soup->p[i+1];
     tri[1].x = soup->p[i+2]; tri[1].y = soup->p[i+3];
     tri[2].x = soup->p[i+4]; tri[2].y = soup->p[i+5];
-    if(c->operation == 0) { /* AND */
+    if(operation == 0) { /* AND */
       size_t m = 3;
       memcpy(buf0, tri, sizeof(tri));
       for(k = 0; k < n && m; ++k) {
         m = clip_half(buf0, m, poly[k], poly[(k+1)%n], ccw, buf1);
         memcpy(buf0, buf1, m * sizeof(p2));
       }
-      if(emit_fan(&out, buf0, m)) { free(poly); free(out.p); return -1; }
+      if(emit_fan(out, buf0, m)) res = -1;
     } else { /* SUB: tri minus polygon = union over edges of the pieces
                 outside edge k and inside edges 0..k-1 */
-      for(k = 0; k < n; ++k) {
+      for(k = 0; k < n && !res; ++k) {
         size_t m = 3, j;
         memcpy(buf0, tri, sizeof(tri));
         m = clip_half(buf0, m, poly[k], poly[(k+1)%n], -ccw, buf1);
@@ -148,11 +145,91 @@ static int carve(struct dvec* soup, const struct ocarving* c)
           m = clip_half(buf0, m, poly[j], poly[(j+1)%n], ccw, buf1);
           memcpy(buf0, buf1, m * sizeof(p2));
         }
-        if(emit_fan(&out, buf0, m)) { free(poly); free(out.p); return -1; }
+        if(emit_fan(out, buf0, m)) res = -1;
       }
     }
   }
+  free(buf0);
+  free(buf1);
+  return res;
+}
+
+/* Split a simple polygon into triangles by ear clipping: repeatedly cut off
+ * a convex vertex whose triangle holds no other vertex (Meisters 1975).
+ * `tris` receives 3 p2 per triangle, counter-clockwise; returns the count,
+ * or -1 when no ear is found (self-intersecting input). */
+static long ear_clip(const p2* in, size_t n, p2* tris)
+{
+  size_t* idx = malloc(n * sizeof(size_t));
+  size_t m = n, i, ntri = 0;
+  const int ccw = poly_area2(in, n) > 0;
+  if(!idx) return -1;
+  for(i = 0; i < n; ++i) idx[i] = ccw ? i : n - 1 - i;
+  while(m > 3) {
+    int cut = 0;
+    for(i = 0; i < m && !cut; ++i) {
+      const p2 a = in[idx[(i + m - 1) % m]], b = in[idx[i]], c = in[idx[(i + 1) % m]];
+      size_t j;
+      int empty = 1;
+      if(edge_side(a, b, c) <= 0) continue; /* reflex or flat corner */
+      for(j = 0; j < m && empty; ++j) {
+        const p2 q = in[idx[j]];
+        if(j == i || j == (i + 1) % m || j == (i + m - 1) % m) continue;
+        if(edge_side(a, b, q) >= 0 && edge_side(b, c, q) >= 0 && edge_side(c, a, q) >= 0)
+          empty = 0;
+      }
+      if(!empty) continue;
+      tris[3*ntri] = a; tris[3*ntri+1] = b; tris[3*ntri+2] = c;
+      ++ntri;
+      memmove(idx + i, idx + i + 1, (m - i - 1) * sizeof(size_t));
+      --m;
+      cut = 1;
+    }
+    if(!cut) { free(idx); return -1; }
+  }
+  if(edge_side(in[idx[0]], in[idx[1]], in[idx[2]]) > 0) {
+    tris[3*ntri] = in[idx[0]]; tris[3*ntri+1] = in[idx[1]]; tris[3*ntri+2] = in[idx[2]];
+    ++ntri;
+  }
+  free(idx);
+  return (long)ntri;
+}
+
+/* Apply one carving polygon to the soup (src/ssol_shape.c:579-652, where
+ * Star-CPR clips). A non-convex polygon is split into triangles: AND keeps
+ * the union of the parts inside each triangle (they do not overlap), SUB
+ * removes the triangles one after the other. */
+static int carve(struct dvec* soup, const struct ocarving* c)
+{
+  struct dvec out = {0};
+  p2* poly;
+  size_t i, n = c->nverts;
+  int res = 0;
+  if(n < 3) { if(c->operation == 0) soup->n = 0; return 0; }
+  poly = malloc(n * sizeof(p2));
+  if(!poly) return -1;
+  for(i = 0; i < n; ++i) { poly[i].x = c->xy[2*i]; poly[i].y = c->xy[2*i+1]; }
+  if(poly_is_convex(poly, n)) {
+    res = carve_convex(soup, poly, n, poly_area2(poly, n) > 0 ? 1 : -1, c->operation, &out);
+  } else {
+    p2* tris = malloc(3 * n * sizeof(p2));
+    long k, ntri = tris ? ear_clip(poly, n, tris) : -1;
+    if(ntri < 0) res = -2;
+    for(k = 0; k < ntri && !res; ++k) {
+      if(c->operation == 0) {
+        res = carve_convex(soup, tris + 3*k, 3, 1, 0, &out);
+      } else {
+        struct dvec rest = {0};
+        res = carve_convex(soup, tris + 3*k, 3, 1, 1, &rest);
+        free(soup->p);
+        *soup = rest;
+      }
+    }
+    free(tris);
+    if(!res && c->operation != 0) { free(poly); return 0; } /* soup already carved */
+  }
   free(poly);
+  if(res) { free(out.p); return res; }
   free(soup->p);
   *soup = out;
   return 0;
@@ -195,13 +272,109 @@ static int build_mesh(struct oshape_geo* g, const struct oshape* s)
   return 0;
 }
 
+/* Carve the 2D sheet, lift it onto the quadric and place it in object
+ * space; the sampling proxy keeps the flat sheet */
+static int carve_and_lift(struct oshape_geo* g, const struct oshape* s,
+  struct dvec soup)
+{
+  size_t i, k;
+  int res;
+  /* Carvings, in order (src/ssol_shape.c:579-652) */
+  for(k = 0; k < s->ncarvings; ++k) {
+    res = carve(&soup, s->carvings + k);
+    if(res) { free(soup.p); return res; }
+  }
+
+  /* Lift to the quadric and to object space, rounded to float like the
+   * reference's Star-3D vertex buffers (src/ssol_shape.c:187-338) */
+  g->ntris = (unsigned)(soup.n / 6);
+  g->nverts = g->ntris * 3;
+  g->rt_verts = malloc(sizeof(double) * 3 * g->nverts);
+  g->samp_verts = malloc(sizeof(double) * 3 * g->nverts);
+  g->rt_ids = malloc(sizeof(unsigned) * 3 * g->ntris);
+  if(!g->rt_verts || !g->samp_verts || !g->rt_ids) { free(soup.p); return -1; }
+  for(i = 0; i < g->nverts; ++i) {
+    double p[3], q[3], xy[2];
+    int c;
+    xy[0] = soup.p[2*i];
+    xy[1] = soup.p[2*i+1];
+    if(s->quadric == OQ_PLANE) {
+      p[0] = (float)xy[0]; p[1] = (float)xy[1]; p[2] = 0;
+    } else {
+      p[0] = xy[0]; p[1] = xy[1]; p[2] = oquadric_z(s->quadric, g->qp, xy);
+    }
+    ov_mat_mul(q, g->qR, p);
+    for(c = 0; c < 3; ++c) g->rt_verts[3*i+c] = (float)(q[c] + g->qT[c]);
+    p[0] = (float)xy[0]; p[1] = (float)xy[1]; p[2] = 0;
+    ov_mat_mul(q, g->qR, p);
+    for(c = 0; c < 3; ++c) g->samp_verts[3*i+c] = (float)(q[c] + g->qT[c]);
+  }
+  for(i = 0; i < 3 * (size_t)g->ntris; ++i) g->rt_ids[i] = (unsigned)i;
+  free(soup.p);
+
+  g->rt_area = g->samp_area = 0;
+  for(i = 0; i < g->ntris; ++i) {
+    g->rt_area += tri_area(g->rt_verts + 9*i, g->rt_verts + 9*i + 3,
+      g->rt_verts + 9*i + 6);
+    g->samp_area += tri_area(g->samp_verts + 9*i, g->samp_verts + 9*i + 3,
+      g->samp_verts + 9*i + 6);
+  }
+  g->rt_area *= 0.5;
+  g->samp_area *= 0.5;
+  return 0;
+}
+
+
+/* Hemisphere (src/ssol_shape.c:1646-1671): the sheet is the xy projection
+ * of a Star-3DUT hemisphere mesh (src/ssol_shape.c:437-489) of the carving
+ * radius, with nslices sectors and nslices stacks; Star-3DUT 0.3.1 is not in
+ * the image, so the mesh is rebuilt here from that description: a pole and
+ * rings k = 1..n at polar angles k pi / (2n), i.e. radius r sin(k pi / 2n),
+ * each of n vertices at azimuths 2 pi i / n */
+static int build_hemisphere_disk(struct oshape_geo* g, const struct oshape* s)
+{
+  double radius = g->qp[0], r2 = -DBL_MAX;
+  size_t n, k, i;
+  struct dvec soup = {0};
+  for(k = 0; k < s->ncarvings; ++k) {
+    if(s->carvings[k].operation != 0) continue;
+    for(i = 0; i < s->carvings[k].nverts; ++i) {
+      const double x = s->carvings[k].xy[2*i], y = s->carvings[k].xy[2*i+1];
+      if(x*x + y*y > r2) r2 = x*x + y*y;
+    }
+  }
+  if(s->ncarvings && r2 >= 0 && sqrt(r2) < radius) radius = sqrt(r2);
+  n = s->slices_hint != SIZE_MAX ? s->slices_hint
+    : (size_t)(5.5 + 11 * radius / g->qp[0]);
+  if(!n) return -2;
+  for(k = 0; k < n; ++k) {
+    const double ra = radius * sin((double)k * M_PI / (2.0 * (double)n));
+    const double rb = radius * sin((double)(k+1) * M_PI / (2.0 * (double)n));
+    for(i = 0; i < n; ++i) {
+      const double a0 = 2 * M_PI * (double)i / (double)n;
+      const double a1 = 2 * M_PI * (double)((i + 1) % n) / (double)n;
+      const double x00 = ra*cos(a0), y00 = ra*sin(a0), x01 = ra*cos(a1), y01 = ra*sin(a1);
+      const double x10 = rb*cos(a0), y10 = rb*sin(a0), x11 = rb*cos(a1), y11 = rb*sin(a1);
+      /* clockwise seen from +Z, like the plane grid */
+      dvec_push(&soup, x00); dvec_push(&soup, y00);
+      dvec_push(&soup, x11); dvec_push(&soup, y11);
+      dvec_push(&soup, x10); dvec_push(&soup, y10);
+      if(k) {
+        dvec_push(&soup, x00); dvec_push(&soup, y00);
+        dvec_push(&soup, x01); dvec_push(&soup, y01);
+        dvec_push(&soup, x11); dvec_push(&soup, y11);
+      }
+    }
+  }
+  return carve_and_lift(g, s, soup);
+}
+
 static int build_punched(struct oshape_geo* g, const struct oshape* s)
 {
   double lower[2] = { DBL_MAX, DBL_MAX }, upper[2] = { -DBL_MAX, -DBL_MAX };
   double size[2], size_min, delta;
   size_t nslices, nsteps[2], nv[2], ix, iy, i, k;
   struct dvec soup = {0};
-  int res;
 
   g->type = 1;
   g->quadric = s->quadric;
@@ -209,7 +382,7 @@ static int build_punched(struct oshape_geo* g, const struct oshape* s)
   memcpy(g->qR, s->qtransform, 9 * sizeof(double));
   memcpy(g->qT, s->qtransform + 9, 3 * sizeof(double));
 
-  if(s->quadric == OQ_HEMISPHERE) return -3; /* needs Star-3DUT: unsupported */
+  if(s->quadric == OQ_HEMISPHERE) return build_hemisphere_disk(g, s);
   if(!s->ncarvings) return -2;
 
   /* AABB of the AND carvings (src/ssol_shape.c:386-409) */
@@ -271,49 +444,7 @@ static int build_punched(struct oshape_geo* g, const struct oshape* s)
   }
   (void)nv;
 
-  /* Carvings, in order (src/ssol_shape.c:579-652) */
-  for(k = 0; k < s->ncarvings; ++k) {
-    res = carve(&soup, s->carvings + k);
-    if(res) { free(soup.p); return res; }
-  }
-
-  /* Lift to the quadric and to object space, rounded to float like the
-   * reference's Star-3D vertex buffers (src/ssol_shape.c:187-338) */
-  g->ntris = (unsigned)(soup.n / 6);
-  g->nverts = g->ntris * 3;
-  g->rt_verts = malloc(sizeof(double) * 3 * g->nverts);
-  g->samp_verts = malloc(sizeof(double) * 3 * g->nverts);
-  g->rt_ids = malloc(sizeof(unsigned) * 3 * g->ntris);
-  if(!g->rt_verts || !g->samp_verts || !g->rt_ids) { free(soup.p); return -1; }
-  for(i = 0; i < g->nverts; ++i) {
-    double p[3], q[3], xy[2];
-    int c;
-    xy[0] = soup.p[2*i];
-    xy[1] = soup.p[2*i+1];
-    if(s->quadric == OQ_PLANE) {
-      p[0] = (float)xy[0]; p[1] = (float)xy[1]; p[2] = 0;
-    } else {
-      p[0] = xy[0]; p[1] = xy[1]; p[2] = oquadric_z(s->quadric, g->qp, xy);
-    }
-    ov_mat_mul(q, g->qR, p);
-    for(c = 0; c < 3; ++c) g->rt_verts[3*i+c] = (float)(q[c] + g->qT[c]);
-    p[0] = (float)xy[0]; p[1] = (float)xy[1]; p[2] = 0;
-    ov_mat_mul(q, g->qR, p);
-    for(c = 0; c < 3; ++c) g->samp_verts[3*i+c] = (float)(q[c] + g->qT[c]);
-  }
-  for(i = 0; i < 3 * (size_t)g->ntris; ++i) g->rt_ids[i] = (unsigned)i;
-  free(soup.p);
-
-  g->rt_area = g->samp_area = 0;
-  for(i = 0; i < g->ntris; ++i) {
-    g->rt_area += tri_area(g->rt_verts + 9*i, g->rt_verts + 9*i + 3,
-      g->rt_verts + 9*i + 6);
-    g->samp_area += tri_area(g->samp_verts + 9*i, g->samp_verts + 9*i + 3,
-      g->samp_verts + 9*i + 6);
-  }
-  g->rt_area *= 0.5;
-  g->samp_area *= 0.5;
-  return 0;
+  return carve_and_lift(g, s, soup);
 }
 
 /*******************************************************************************

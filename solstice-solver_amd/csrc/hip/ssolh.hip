@@ -1660,6 +1660,8 @@ struct ssolh_context {
   uint64_t segments = 0;
   double kernel_ms = 0, trace_ms = 0, shade_ms = 0;
   uint32_t iterations = 0;
+  uint32_t wavefronts = 0;    // of the last solve
+  uint32_t refilled = 0;
   // grid of each looping kernel: its workgroups resident at once
   std::vector<std::pair<const void*, uint32_t>> resident;
 };
@@ -1979,19 +1981,34 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   if (trk) graph_iters = 0;
   else c->trk_paths = 0;  // the recorded paths belong to the last solve only
   if (npend > MAX_PEND) { set_err("ssolh_solve: at most %d pending entries", MAX_PEND); return SSOLH_ERR_LIMIT; }
+  // Tally layout: each tally is striped over up to 256 copies while its
+  // stripes hold at most 2^21 doubles
+  auto pow2_stripes = [](size_t per_stripe_doubles) {
+    size_t k = 256;
+    while (k > 1 && k * per_stripe_doubles > (size_t(1) << 21)) k >>= 1;
+    return (uint32_t)k;
+  };
+  const uint32_t sg = pow2_stripes((size_t)njobs * 16), sr = pow2_stripes((size_t)c->nrcv * 40),
+                 ss = pow2_stripes((size_t)c->nsamp * 5),
+                 sx = pow2_stripes((size_t)c->nsamp * c->nrcv * 40),
+                 sp = pow2_stripes((size_t)c->nprim * 40);
+  const size_t n_global = (size_t)sg * njobs * 16, n_rcv = (size_t)sr * c->nrcv * 40,
+               n_samp = (size_t)ss * c->nsamp * 4;
+  const size_t n_sxr = (size_t)sx * c->nsamp * c->nrcv * 40, n_prim = (size_t)sp * c->nprim * 40;
+  const size_t n_u64 = (size_t)ss * c->nsamp;
+  const size_t tally_bytes = (n_global + n_rcv + n_samp + n_sxr + n_prim + n_u64) * 8 + 64;
   // Wavefront width. By default every realisation is in flight at once when
   // 2/5 of the device memory holds their state: the realisations then start
   // together and each iteration traces one generation of rays (shadow rays,
   // first reflections, ...), which keeps the waves coherent; larger solves
   // run as many as fit, each slot taking its next realisation in place.
   uint64_t slots64 = path_slots;
+  const size_t per_slot = 6 * 4 * (fp32 ? sizeof(float) : sizeof(double)) + 3 * sizeof(uint4) +
+                          (size_t)npend * 2 * (sizeof(uint32_t) + 10 * (fp32 ? sizeof(float) : sizeof(double))) +
+                          (fp32 ? sizeof(RayRec32) + sizeof(HitRec32) : sizeof(RayRec) + sizeof(HitRec)) +
+                          2 * sizeof(uint32_t) + 1 + (adj ? sizeof(double4) + sizeof(uint32_t) : 0) +
+                          (trk ? sizeof(uint32_t) : 0);
   if (!slots64) {
-    const size_t real = fp32 ? sizeof(float) : sizeof(double);
-    const size_t per_slot = 6 * 4 * real + 3 * sizeof(uint4) +
-                            (size_t)npend * 2 * (sizeof(uint32_t) + 10 * real) +
-                            (fp32 ? sizeof(RayRec32) + sizeof(HitRec32) : sizeof(RayRec) + sizeof(HitRec)) +
-                            2 * sizeof(uint32_t) + 1 + (adj ? sizeof(double4) + sizeof(uint32_t) : 0) +
-                            (trk ? sizeof(uint32_t) : 0);
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const uint64_t fit = (uint64_t)(total_b / 5 * 2) / per_slot;
@@ -2123,20 +2140,6 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   }
 
   // tallies (striped where many waves meet, see Tallies)
-  auto pow2_stripes = [](size_t per_stripe_doubles) {
-    size_t k = 256;
-    while (k > 1 && k * per_stripe_doubles > (size_t(1) << 21)) k >>= 1;
-    return (uint32_t)k;
-  };
-  const uint32_t sg = pow2_stripes((size_t)njobs * 16), sr = pow2_stripes((size_t)c->nrcv * 40),
-                 ss = pow2_stripes((size_t)c->nsamp * 5),
-                 sx = pow2_stripes((size_t)c->nsamp * c->nrcv * 40),
-                 sp = pow2_stripes((size_t)c->nprim * 40);
-  const size_t n_global = (size_t)sg * njobs * 16, n_rcv = (size_t)sr * c->nrcv * 40,
-               n_samp = (size_t)ss * c->nsamp * 4;
-  const size_t n_sxr = (size_t)sx * c->nsamp * c->nrcv * 40, n_prim = (size_t)sp * c->nprim * 40;
-  const size_t n_u64 = (size_t)ss * c->nsamp;
-  const size_t tally_bytes = (n_global + n_rcv + n_samp + n_sxr + n_prim + n_u64) * 8 + 64;
   if ((r = dev_alloc(c->tallies, tally_bytes))) return r;
   c->tally_bytes = tally_bytes;
   double* tb = (double*)c->tallies.p;
@@ -2337,13 +2340,17 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     return 0;
   };
 
-  uint32_t iter = 0;
+  // iterations of the whole solve (reported) and of the current wavefront
+  // (bounded: a wavefront drains within its deepest path, which roulette
+  // ends with probability 1 - 2^-k after 16k bounces past depth 65)
+  uint32_t iter = 0, wave_iter = 0;
+  constexpr uint32_t MAX_WAVE_ITER = 1000000u;
   // host-driven iterations until the list is empty
   auto drain_host = [&](uint32_t count) -> int {
     int cur = 0;
     while (count) {
-      if (iter >= 1000000u) {
-        set_err("ssolh_solve: the wavefront did not drain after %u iterations", iter);
+      if (wave_iter >= MAX_WAVE_ITER) {
+        set_err("ssolh_solve: the wavefront did not drain after %u iterations", wave_iter);
         return SSOLH_ERR_LIMIT;
       }
       HIP_TRY(hipEventRecord(c->ev[1], c->stream));
@@ -2363,6 +2370,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       cur ^= 1;
       count = total;
       ++iter;
+      ++wave_iter;
     }
     return 0;
   };
@@ -2376,8 +2384,8 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   hipGraphExec_t exec = nullptr;
   auto drain_graph = [&](uint32_t count) -> int {
     while (count) {
-      if (iter >= 1000000u) {
-        set_err("ssolh_solve: the wavefront did not drain after %u iterations", iter);
+      if (wave_iter >= MAX_WAVE_ITER) {
+        set_err("ssolh_solve: the wavefront did not drain after %u iterations", wave_iter);
         return SSOLH_ERR_LIMIT;
       }
       const hipError_t le = hipGraphLaunch(exec, c->stream);
@@ -2388,14 +2396,18 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       if (se != hipSuccess) { set_err("graph solve: %s", hipGetErrorString(se)); return -(int)se; }
       count = total;
       iter += gi;
+      wave_iter += gi;
     }
     return 0;
   };
 
   const uint64_t chunk = chunked ? slots : N;  // realisations started per wavefront
   int gerr = 0;
+  uint32_t waves = 0;
   for (uint64_t first = 0; first < N && !gerr; first += chunk) {
     const uint32_t count = (uint32_t)std::min<uint64_t>(N - first, slots);
+    wave_iter = 0;
+    ++waves;
     if ((gerr = start(first, count))) break;
     if (!graph_iters) {
       gerr = drain_host(count);
@@ -2415,7 +2427,15 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       }
       gerr = drain_graph(count);
     }
-    if (!chunked) break;  // refill: one wavefront runs every realisation
+    if (!chunked || gerr) break;  // refill: one wavefront runs every realisation
+    // Too many failed paths: like the reference's loop, which skips every
+    // realisation not yet started once the limit is hit
+    // (src/ssol_solver.c:1227, 1241-1245), no further chunk starts. The
+    // stream is idle here (the drain synchronised it).
+    unsigned aborted = 0;
+    const hipError_t ae = hipMemcpy(&aborted, &ctr->aborted, sizeof(aborted), hipMemcpyDeviceToHost);
+    if (ae != hipSuccess) { set_err("ssolh_solve: %s", hipGetErrorString(ae)); gerr = -(int)ae; break; }
+    if (aborted) break;
   }
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
@@ -2445,6 +2465,8 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   c->trace_ms = ms_trace;
   c->shade_ms = ms_shade;
   c->iterations = iter;
+  c->wavefronts = waves;
+  c->refilled = slots < N && !chunked;
   return 0;
 }
 
@@ -2597,6 +2619,8 @@ SSOLH_API int ssolh_tallies_download(ssolh_context* c, struct ssolh_tallies* o) 
   o->pending_spills = c->last.spills;
   o->aborted = c->last.aborted;
   o->iterations = c->iterations;
+  o->wavefronts = c->wavefronts;
+  o->refilled = c->refilled;
   o->kernel_ms = c->kernel_ms;
   o->trace_ms = c->trace_ms;
   o->shade_ms = c->shade_ms;

@@ -234,18 +234,24 @@ res_T punched_setup(ssol_shape* shape, const ssol_punched_surface* ps) {
     nslices = q.slices_count_hint != SIZE_MAX ? q.slices_count_hint
                                              : (size_t)(5.5 + 11 * radius / p[0]);
     if (!nslices) return RES_BAD_ARG;
-    // polar grid: nslices rings x 4*nslices sectors
-    const size_t nsec = 4 * nslices;
-    for (size_t ir = 0; ir < nslices; ++ir) {
-      const double r0 = radius * (double)ir / (double)nslices;
-      const double r1 = radius * (double)(ir + 1) / (double)nslices;
-      for (size_t is = 0; is < nsec; ++is) {
-        const double a0 = 2 * M_PI * (double)is / (double)nsec;
-        const double a1 = 2 * M_PI * (double)(is + 1) / (double)nsec;
-        const P2 p00{r0 * std::cos(a0), r0 * std::sin(a0)}, p01{r0 * std::cos(a1), r0 * std::sin(a1)};
-        const P2 p10{r1 * std::cos(a0), r1 * std::sin(a0)}, p11{r1 * std::cos(a1), r1 * std::sin(a1)};
+    // Triangulated disk: the xy projection of a hemisphere mesh of the
+    // carving radius with nslices sectors around the pole and nslices
+    // stacks from the pole to the equator (Star-3DUT's hemisphere, which
+    // the reference projects, src/ssol_shape.c:437-489): ring k at radius
+    // r sin(k pi / (2 nslices)), so the outer ring is the regular
+    // nslices-gon inscribed in the carving circle
+    const size_t n = nslices;
+    auto ring_pt = [&](size_t k, size_t i) {
+      const double rr = radius * std::sin((double)k * M_PI / (2.0 * (double)n));
+      const double a = 2 * M_PI * (double)(i % n) / (double)n;
+      return P2{rr * std::cos(a), rr * std::sin(a)};
+    };
+    for (size_t k = 0; k < n; ++k) {
+      for (size_t i = 0; i < n; ++i) {
+        const P2 p00 = ring_pt(k, i), p01 = ring_pt(k, i + 1);
+        const P2 p10 = ring_pt(k + 1, i), p11 = ring_pt(k + 1, i + 1);
         // clockwise seen from +Z, like the plane grid
-        if (ir == 0) {
+        if (k == 0) {
           soup.push_back({p00, p11, p10});
         } else {
           soup.push_back({p00, p11, p10});

@@ -110,20 +110,34 @@ uint32_t medium_id(Flat& f, const ssol_medium& m) {
   return (uint32_t)(f.media.size() - 1);
 }
 
+// The k-th of the three surface fragments shaders are baked on
+ssol_surface_fragment probe_fragment(int k) {
+  ssol_surface_fragment fr = SSOL_SURFACE_FRAGMENT_NULL;
+  const double a = 0.3 + 0.7 * k;
+  fr.dir[0] = 0.1 * k; fr.dir[1] = -0.2; fr.dir[2] = -1;
+  fr.P[0] = 1.5 * k; fr.P[1] = -2.0 * k; fr.P[2] = 0.25 * k;
+  fr.Ng[0] = 0; fr.Ng[1] = std::sin(a); fr.Ng[2] = std::cos(a);
+  fr.Ns[0] = std::sin(a); fr.Ns[1] = 0; fr.Ns[2] = std::cos(a);
+  fr.uv[0] = 0.25 * k; fr.uv[1] = 1 - 0.25 * k;
+  fr.dPdu[0] = 1; fr.dPdv[1] = 1;
+  return fr;
+}
+
+// The wavelengths shaders are baked at: the sun spectrum's, or its first,
+// middle and last when it has more than three
+std::vector<double> probe_wavelengths(const ssol_spectrum* sp) {
+  std::vector<double> wls = sp->wavelengths;
+  if (wls.size() > 3) wls = {wls.front(), wls[wls.size() / 2], wls.back()};
+  return wls;
+}
+
 // Evaluate a shader on probe fragments; require a constant value.
 bool bake_scalar(ssol_device* dev, const ssol_material* m, ssol_shader_getter_T fn,
                  const std::vector<double>& wls, double* out, const char* what) {
   double first = 0;
   bool have = false;
   for (int k = 0; k < 3; ++k) {
-    ssol_surface_fragment fr = SSOL_SURFACE_FRAGMENT_NULL;
-    const double a = 0.3 + 0.7 * k;
-    fr.dir[0] = 0.1 * k; fr.dir[1] = -0.2; fr.dir[2] = -1;
-    fr.P[0] = 1.5 * k; fr.P[1] = -2.0 * k; fr.P[2] = 0.25 * k;
-    fr.Ng[0] = 0; fr.Ng[1] = std::sin(a); fr.Ng[2] = std::cos(a);
-    fr.Ns[0] = std::sin(a); fr.Ns[1] = 0; fr.Ns[2] = std::cos(a);
-    fr.uv[0] = 0.25 * k; fr.uv[1] = 1 - 0.25 * k;
-    fr.dPdu[0] = 1; fr.dPdv[1] = 1;
+    const ssol_surface_fragment fr = probe_fragment(k);
     for (double wl : wls) {
       double v = 0;
       fn(dev, m->buf, wl, &fr, &v);
@@ -431,6 +445,36 @@ void push_medium_sig(std::vector<uint64_t>& sig, const ssol_medium& m) {
   push_data_sig(sig, m.refractive_index);
 }
 
+// What a material's shaders return on the probes they are baked on: the
+// shaders read their parameters from the material's parameter buffer (or
+// anywhere else), which a client may rewrite between solves without any
+// setter call, so the baked constants are checked against a fresh
+// evaluation on every solve
+void push_shader_sig(std::vector<uint64_t>& sig, const ssol_material* m,
+                     const std::vector<double>& wls) {
+  auto push = [&](double v) {
+    uint64_t b;
+    memcpy(&b, &v, 8);
+    sig.push_back(b);
+  };
+  for (int k = 0; k < 3; ++k) {
+    const ssol_surface_fragment fr = probe_fragment(k);
+    if (m->normal) {
+      double v[3] = {0, 0, 0};
+      m->normal(m->dev, m->buf, wls.empty() ? 500 : wls.front(), &fr, v);
+      push(v[0]); push(v[1]); push(v[2]);
+    }
+    for (ssol_shader_getter_T fn : {m->reflectivity, m->roughness}) {
+      if (!fn) continue;
+      for (double wl : wls) {
+        double v = 0;
+        fn(m->dev, m->buf, wl, &fr, &v);
+        push(v);
+      }
+    }
+  }
+}
+
 // Everything flatten() reads, so that an unchanged scene is not flattened
 // and uploaded again
 // Instance transforms go to *xf, the rest to the returned signature: scenes
@@ -449,6 +493,7 @@ std::vector<uint64_t> scene_signature(const ssol_scene* scn, std::vector<uint64_
     sig.push_back(scn->atmosphere->gen);
     push_data_sig(sig, scn->atmosphere->extinction);
   }
+  std::vector<const ssol_material*> mats;
   for (const ssol_instance* in : scn->instances) {
     P(in);
     sig.push_back(in->id);
@@ -471,8 +516,16 @@ std::vector<uint64_t> scene_signature(const ssol_scene* scn, std::vector<uint64_
         push_medium_sig(sig, m->in_medium);
         push_medium_sig(sig, m->out_medium);
         push_medium_sig(sig, m->slab_medium);
+        mats.push_back(m);
       }
     }
+  }
+  std::sort(mats.begin(), mats.end());
+  mats.erase(std::unique(mats.begin(), mats.end()), mats.end());
+  if (scn->sun && scn->sun->spectrum) {
+    const std::vector<double> wls = probe_wavelengths(scn->sun->spectrum);
+    for (const ssol_material* m : mats)
+      if (m->type != SSOL_MATERIAL_VIRTUAL) push_shader_sig(sig, m, wls);
   }
   return sig;
 }
@@ -599,8 +652,7 @@ res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
   ssol_device* dev = scn->dev;
   ssol_sun* sun = scn->sun;
   // wavelengths at which shaders are probed
-  std::vector<double> wls = sun->spectrum->wavelengths;
-  if (wls.size() > 3) wls = {wls.front(), wls[wls.size() / 2], wls.back()};
+  const std::vector<double> wls = probe_wavelengths(sun->spectrum);
 
   // air (src/ssol_solver.c:660-664)
   ssol_medium air = SSOL_MEDIUM_VACUUM;
@@ -995,6 +1047,8 @@ res_T ssol_solve(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t
   est->stats.trace_ms = t.trace_ms;
   est->stats.shade_ms = t.shade_ms;
   est->stats.iterations = t.iterations;
+  est->stats.wavefronts = t.wavefronts;
+  est->stats.refilled = t.refilled;
   // RNG state after the run: the next realisations' counters
   ssp_rng* nr = new ssp_rng();
   nr->type = rng->type;

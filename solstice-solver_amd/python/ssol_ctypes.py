@@ -157,7 +157,7 @@ class AutotuneResult(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("segments", C.c_uint64), ("paths", C.c_uint64), ("pending_spills", C.c_uint64),
                 ("kernel_ms", C.c_double), ("trace_ms", C.c_double), ("shade_ms", C.c_double),
-                ("iterations", C.c_uint)]
+                ("iterations", C.c_uint), ("wavefronts", C.c_uint), ("refilled", C.c_uint)]
 
 
 class PathTracker(C.Structure):

@@ -148,3 +148,19 @@ def test_threefry_export_matches_oracle():
         out = A()
         lib.ssp_threefry4x64_20(A(*ctr), A(*key), out)
         assert list(out) == O.threefry(ctr, key)
+
+
+def test_c_client_links_against_the_headers_and_libssol():
+    """The C client (tests/c_client/config1_client.c) compiles against
+    include/ssol.h and include/ssp.h alone and resolves every symbol from
+    lib/libssol.so; without a GPU it builds its scene through the C API and
+    stops at ssol_solve with an error (no CPU fallback)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    client = os.path.join(root, "tests", "c_client", "config1_client")
+    assert os.path.exists(client), "build the client with __graft_entry__.build()"
+    ldd = subprocess.run(["ldd", client], capture_output=True, text=True).stdout
+    assert "libssol.so =>" in ldd and "not found" not in ldd, ldd
+    p = subprocess.run([client, "1000"], capture_output=True, text=True, timeout=120)
+    if p.returncode != 0:
+        assert p.returncode == 2 and "ssol_solve" in p.stderr, (p.returncode, p.stderr)

@@ -67,8 +67,8 @@ def _compare(spec, n, seed_gpu=3, seed_orc=4, prim_maps=True, sampled=True):
                     for prim in range(lib.shape_ntris(shape)):
                         gp = g.primitive(inst, shape, prim, side)
                         op = o.primitive(inst, s_idx, prim, side)
-                        maps.append((gp["incoming_flux"], op["incoming_flux"]))
-                        maps.append((gp["absorbed_flux"], op["absorbed_flux"]))
+                        for f in L.RECEIVER_FIELDS:
+                            maps.append((gp[f], op[f]))
     assert not bad, bad[:6]
     if maps:
         mz2, zmax, m = _z_stats(maps, min_hits=20)

@@ -63,6 +63,7 @@ QUADRICS = [
     (2, (5.0, 20.0)),         # paraboloid with two focal lengths
     (3, (1.0, 9.0)),          # hyperboloid img 1, real 9
     (4, (3.0, 0.0)),          # parabolic cylinder
+    (5, (10.0, 0.0)),         # hemisphere radius 10
 ]
 
 
@@ -79,6 +80,8 @@ def _qparam(q, data):
         a2 = g * g * (f - f * f)
         b = g * abs(f - 0.5)
         return [g / 2, a2 / (b * b), 1 / a2, b]
+    if q == 5:
+        return [data[0], data[0] * data[0], 0, 0]
     return [0, 0, 0, 0]
 
 

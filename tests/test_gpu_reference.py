@@ -110,7 +110,7 @@ def compare(scene, n, seed_gpu=11, seed_orc=12, k=3.0):
                         for prim in range(ntris):
                             gp = g.primitive(inst, s_idx, prim, side)
                             op = o.primitive(inst, s_idx, prim, side)
-                            for f in ("incoming_flux", "absorbed_flux"):
+                            for f in L.RECEIVER_FIELDS:
                                 chk(f"prim[{prim},{side}].{f}", gp[f], op[f])
         if inst.sample:
             gs, os_ = g.sampled(inst), o.sampled(inst)
@@ -122,7 +122,8 @@ def compare(scene, n, seed_gpu=11, seed_orc=12, k=3.0):
                     if rcv.receiver_mask & side:
                         gx = g.sampled_x_receiver(inst, rcv, side)
                         ox = o.sampled_x_receiver(inst, rcv, side)
-                        chk("sxr.incoming", gx["incoming_flux"], ox["incoming_flux"])
+                        for f in L.RECEIVER_FIELDS:
+                            chk(f"sxr.{f}", gx[f], ox[f])
     assert not bad, bad[:6]
 
 
