@@ -626,6 +626,11 @@ SSOL_API res_T ssol_device_set_precision
 SSOL_API res_T ssol_device_set_wavefront_size
   (struct ssol_device* dev, const size_t path_slots);
 
+/* The device's GPU context (an ssolh_context*, include/ssolh.h), created by
+ * its first solve and holding the scene that solve uploaded: for calling the
+ * device C-ABI directly. RES_BAD_OP before the first solve. */
+SSOL_API res_T ssol_device_get_gpu_context(const struct ssol_device* dev, void** ctx);
+
 /* Counters of the last solve: traced ray segments (including shadow rays)
  * and GPU kernel time in milliseconds. */
 struct ssol_solve_stats {

@@ -406,6 +406,22 @@ SSOLH_API int ssolh_solve_batch
  * from the uploaded scene's tally layout */
 SSOLH_API int ssolh_tallies_download(ssolh_context* ctx, struct ssolh_tallies* out);
 
+/* ssolh_solve followed by ssolh_tallies_download, allocating nothing: no
+ * counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Every buffer of the solve is carved from a 64 MiB
+ * device array that the library holds on each GPU it runs on until the
+ * process ends; the scene uploaded by ssolh_scene_upload is the input and
+ * `out` (host arrays, as for ssolh_tallies_download) the output. The
+ * wavefront is as wide as fits beside the tallies (args->path_slots, when
+ * set, must fit too); a scene whose tallies need more than half of the array
+ * is rejected with SSOLH_ERR_LIMIT and a message. Plain single-sun solves
+ * driven from the host only: args->graph_iterations must be 0. The array is
+ * shared by every call on one GPU and keeps the last call's contents; calls
+ * that overlap on one GPU (two contexts, two threads) race. */
+SSOLH_API int ssolh_solve_noalloc
+  (ssolh_context* ctx, const struct ssolh_solve_args* args, struct ssolh_tallies* out);
+
 /* Deterministic kernels, one launch over n inputs each, for known-answer
  * tests against the oracle. Inputs and outputs are host arrays. */
 SSOLH_API int ssolh_test_quadric_hits

@@ -35,6 +35,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "ssolh.h"
@@ -1628,9 +1629,23 @@ struct DevBuf {
   size_t n = 0;
 };
 
+// Device memory of ssolh_solve_noalloc: every buffer of its solve (path
+// state, rays, hits, work lists, scan arrays, counters, tallies) is carved
+// from this array, which each GPU holds from the library's load to the end
+// of the process. Host code never names it as an address: each device's own
+// copy is reached through the pointer hipGetSymbolAddress returns with that
+// device current (kept per device by ssolh_create).
+constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[MAX_DEVICES] = {};
+
 struct ssolh_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  void* scratch = nullptr;  // this device's copy of the scratch array
   hipEvent_t ev[4] = {};
   // scene
   std::vector<DevBuf> scene_bufs;
@@ -1703,6 +1718,32 @@ int dev_alloc(DevBuf& b, size_t bytes) {
   if (!bytes) bytes = 16;
   HIP_TRY(hipMalloc(&b.p, bytes));
   b.n = bytes;
+  return 0;
+}
+
+// A solve's buffers carved from caller-independent device memory that is
+// never freed (the library's static scratch array): each take advances a
+// cursor, 256-byte aligned; nothing is allocated or released.
+struct Arena {
+  char* base;
+  size_t size;
+  size_t used;
+};
+
+int arena_take(Arena& a, DevBuf& b, size_t bytes) {
+  if (!bytes) bytes = 16;
+  // taken earlier in this solve and large enough (the start-instance
+  // histogram of each chunk)
+  if (b.p && b.n >= bytes && (char*)b.p >= a.base && (char*)b.p < a.base + a.size) return 0;
+  const size_t off = (a.used + 255) & ~size_t(255);
+  if (off > a.size || bytes > a.size - off) {
+    set_err("ssolh_solve_noalloc: the solve needs more than the %zu bytes of the scratch array "
+            "(%zu in use, %zu more asked)", a.size, off, bytes);
+    return SSOLH_ERR_LIMIT;
+  }
+  b.p = a.base + off;
+  b.n = bytes;
+  a.used = off + bytes;
   return 0;
 }
 
@@ -1797,9 +1838,33 @@ SSOLH_API int ssolh_create(int device, ssolh_context** out) {
             prop.gcnArchName);
     return SSOLH_ERR_NO_DEVICE;
   }
+  if (device < 0 || device >= MAX_DEVICES) {
+    set_err("ssolh_create: device %d beyond the %d this library serves", device, MAX_DEVICES);
+    return SSOLH_ERR_NO_DEVICE;
+  }
+  HIP_TRY(hipSetDevice(device));
+  void* scratch_ptr = nullptr;
+  {
+    // once per device, with the device current: the address of its copy of
+    // the scratch array
+    // (without it only ssolh_solve_noalloc is unavailable, and says so)
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e == hipSuccess && p) {
+        g_scratch[device] = p;
+      } else {
+        (void)hipGetLastError();
+        fprintf(stderr, "ssolh_create: hipGetSymbolAddress(scratch) on device %d: %s\n", device,
+                hipGetErrorString(e));
+      }
+    }
+    scratch_ptr = g_scratch[device];
+  }
   ssolh_context* c = new ssolh_context();
   c->device = device;
-  HIP_TRY(hipSetDevice(device));
+  c->scratch = scratch_ptr;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
   if (int r = init_resident_grids(c, (uint32_t)prop.multiProcessorCount)) {
@@ -1972,7 +2037,16 @@ struct AdjHost {
 int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, uint64_t counter,
               bool fp32, uint32_t path_slots, uint32_t npend, uint32_t njobs,
               const uint64_t* job_first_h, const ssolh_sun* job_suns_h, uint32_t graph_iters,
-              const AdjHost* adj = nullptr, const TrackHost* trk = nullptr) {
+              const AdjHost* adj = nullptr, const TrackHost* trk = nullptr,
+              Arena* arena = nullptr) {
+  // buffers come from the device allocator, or all from the arena
+  auto alloc = [&](DevBuf& b, size_t bytes) -> int {
+    return arena ? arena_take(*arena, b, bytes) : dev_alloc(b, bytes);
+  };
+  if (arena && (adj || trk || job_suns_h || graph_iters)) {
+    set_err("ssolh_solve_noalloc: plain single-sun solves driven from the host only");
+    return SSOLH_ERR_ARG;
+  }
   if (adj && fp32) { set_err("ssolh_solve_adjoint: the adjoint runs in the FP64 form"); return SSOLH_ERR_ARG; }
   if (trk && (fp32 || adj || job_suns_h)) {
     set_err("ssolh_solve_tracked: paths are recorded by plain FP64 solves only");
@@ -1982,21 +2056,35 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   else c->trk_paths = 0;  // the recorded paths belong to the last solve only
   if (npend > MAX_PEND) { set_err("ssolh_solve: at most %d pending entries", MAX_PEND); return SSOLH_ERR_LIMIT; }
   // Tally layout: each tally is striped over up to 256 copies while its
-  // stripes hold at most 2^21 doubles
-  auto pow2_stripes = [](size_t per_stripe_doubles) {
-    size_t k = 256;
-    while (k > 1 && k * per_stripe_doubles > (size_t(1) << 21)) k >>= 1;
-    return (uint32_t)k;
-  };
-  const uint32_t sg = pow2_stripes((size_t)njobs * 16), sr = pow2_stripes((size_t)c->nrcv * 40),
-                 ss = pow2_stripes((size_t)c->nsamp * 5),
-                 sx = pow2_stripes((size_t)c->nsamp * c->nrcv * 40),
-                 sp = pow2_stripes((size_t)c->nprim * 40);
-  const size_t n_global = (size_t)sg * njobs * 16, n_rcv = (size_t)sr * c->nrcv * 40,
-               n_samp = (size_t)ss * c->nsamp * 4;
-  const size_t n_sxr = (size_t)sx * c->nsamp * c->nrcv * 40, n_prim = (size_t)sp * c->nprim * 40;
-  const size_t n_u64 = (size_t)ss * c->nsamp;
-  const size_t tally_bytes = (n_global + n_rcv + n_samp + n_sxr + n_prim + n_u64) * 8 + 64;
+  // stripes hold at most 2^21 doubles; from the scratch array every stripe
+  // count is halved until all tallies take at most half of it.
+  uint32_t sg, sr, ss, sx, sp;
+  size_t n_global, n_rcv, n_samp, n_sxr, n_prim, n_u64, tally_bytes;
+  for (size_t cap = 256;; cap >>= 1) {
+    auto pow2_stripes = [cap](size_t per_stripe_doubles) {
+      size_t k = cap;
+      while (k > 1 && k * per_stripe_doubles > (size_t(1) << 21)) k >>= 1;
+      return (uint32_t)k;
+    };
+    sg = pow2_stripes((size_t)njobs * 16);
+    sr = pow2_stripes((size_t)c->nrcv * 40);
+    ss = pow2_stripes((size_t)c->nsamp * 5);
+    sx = pow2_stripes((size_t)c->nsamp * c->nrcv * 40);
+    sp = pow2_stripes((size_t)c->nprim * 40);
+    n_global = (size_t)sg * njobs * 16;
+    n_rcv = (size_t)sr * c->nrcv * 40;
+    n_samp = (size_t)ss * c->nsamp * 4;
+    n_sxr = (size_t)sx * c->nsamp * c->nrcv * 40;
+    n_prim = (size_t)sp * c->nprim * 40;
+    n_u64 = (size_t)ss * c->nsamp;
+    tally_bytes = (n_global + n_rcv + n_samp + n_sxr + n_prim + n_u64) * 8 + 64;
+    if (!arena || tally_bytes <= arena->size / 2 || cap == 1) break;
+  }
+  if (arena && tally_bytes > arena->size / 2) {
+    set_err("ssolh_solve_noalloc: the scene's tallies take %zu bytes; at most %zu fit in the "
+            "scratch array", tally_bytes, arena->size / 2);
+    return SSOLH_ERR_LIMIT;
+  }
   // Wavefront width. By default every realisation is in flight at once when
   // 2/5 of the device memory holds their state: the realisations then start
   // together and each iteration traces one generation of rays (shadow rays,
@@ -2008,6 +2096,18 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
                           (fp32 ? sizeof(RayRec32) + sizeof(HitRec32) : sizeof(RayRec) + sizeof(HitRec)) +
                           2 * sizeof(uint32_t) + 1 + (adj ? sizeof(double4) + sizeof(uint32_t) : 0) +
                           (trk ? sizeof(uint32_t) : 0);
+  if (!slots64 && arena) {
+    // from the scratch array: what is left beside the tallies, the
+    // per-block scan arrays, counters and start-instance histogram
+    const size_t fixed = tally_bytes + (size_t)c->ninstances * sizeof(uint32_t) + (64u << 10);
+    const size_t rest = arena->size > fixed ? arena->size - fixed : 0;
+    slots64 = rest / (per_slot + 1) / BLOCK * BLOCK;  // + 8 B of per-block counts per block
+    if (slots64 < BLOCK) {
+      set_err("ssolh_solve_noalloc: no path slot fits beside the tallies in the scratch array");
+      return SSOLH_ERR_LIMIT;
+    }
+    if (slots64 > N) slots64 = N;
+  }
   if (!slots64) {
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
@@ -2039,7 +2139,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
                             npend * n * 10 * real, npend * n * 10 * real};
   size_t bytes_state = 0;
   for (size_t sz : sizes) bytes_state += (sz + 255) / 256 * 256;
-  int r = dev_alloc(c->state, bytes_state);
+  int r = alloc(c->state, bytes_state);
   if (r) return r;
   {
     char* p = (char*)c->state.p;
@@ -2072,24 +2172,24 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     c->P32.pval = (float*)arr[11];
     c->P32.qval = (float*)arr[12];
   }
-  if ((r = dev_alloc(c->rays, n * (fp32 ? sizeof(RayRec32) : sizeof(RayRec))))) return r;
-  if ((r = dev_alloc(c->hits, n * (fp32 ? sizeof(HitRec32) : sizeof(HitRec))))) return r;
-  for (int i = 0; i < 2; ++i) if ((r = dev_alloc(c->lists[i], n * sizeof(uint32_t)))) return r;
-  if ((r = dev_alloc(c->emitted, n))) return r;
-  if ((r = dev_alloc(c->block_emits, max_blocks * sizeof(uint32_t)))) return r;
-  if ((r = dev_alloc(c->block_offsets, max_blocks * sizeof(uint32_t)))) return r;
-  if ((r = dev_alloc(c->scan_chunks, ((max_blocks + SCAN_CHUNK - 1) / SCAN_CHUNK) * sizeof(uint32_t))))
+  if ((r = alloc(c->rays, n * (fp32 ? sizeof(RayRec32) : sizeof(RayRec))))) return r;
+  if ((r = alloc(c->hits, n * (fp32 ? sizeof(HitRec32) : sizeof(HitRec))))) return r;
+  for (int i = 0; i < 2; ++i) if ((r = alloc(c->lists[i], n * sizeof(uint32_t)))) return r;
+  if ((r = alloc(c->emitted, n))) return r;
+  if ((r = alloc(c->block_emits, max_blocks * sizeof(uint32_t)))) return r;
+  if ((r = alloc(c->block_offsets, max_blocks * sizeof(uint32_t)))) return r;
+  if ((r = alloc(c->scan_chunks, ((max_blocks + SCAN_CHUNK - 1) / SCAN_CHUNK) * sizeof(uint32_t))))
     return r;
-  if ((r = dev_alloc(c->counters, sizeof(Counters) + 4 * sizeof(unsigned)))) return r;
+  if ((r = alloc(c->counters, sizeof(Counters) + 4 * sizeof(unsigned)))) return r;
   // adjoint buffers
   AdjParams A{};
   if (adj) {
     const size_t ninst = c->ninstances, ngm = (size_t)c->nprim * 2;
     const size_t in_bytes = ninst * sizeof(int32_t) + 64 + (size_t)adj->nparams * 6 * sizeof(double) +
                             ngm * sizeof(double) + 64;
-    if ((r = dev_alloc(c->adj_state, n * (sizeof(double4) + sizeof(uint32_t)) + 256))) return r;
-    if ((r = dev_alloc(c->adj_in, in_bytes))) return r;
-    if ((r = dev_alloc(c->adj_grad, (size_t)adj->nparams * 3 * sizeof(double) + 8))) return r;
+    if ((r = alloc(c->adj_state, n * (sizeof(double4) + sizeof(uint32_t)) + 256))) return r;
+    if ((r = alloc(c->adj_in, in_bytes))) return r;
+    if ((r = alloc(c->adj_grad, (size_t)adj->nparams * 3 * sizeof(double) + 8))) return r;
     char* ai = (char*)c->adj_in.p;
     int32_t* d_ip = (int32_t*)ai;
     double* d_axes = (double*)(ai + (ninst * sizeof(int32_t) + 63) / 64 * 64);
@@ -2114,10 +2214,10 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   TrackParams K{};
   if (trk) {
     const uint64_t cap = std::max<uint64_t>(trk->cap, 1);
-    if ((r = dev_alloc(c->trk_vtx, cap * sizeof(ssolh_track_vertex)))) return r;
-    if ((r = dev_alloc(c->trk_type, N))) return r;
-    if ((r = dev_alloc(c->trk_seq, n * sizeof(uint32_t)))) return r;
-    if ((r = dev_alloc(c->trk_count, sizeof(unsigned long long)))) return r;
+    if ((r = alloc(c->trk_vtx, cap * sizeof(ssolh_track_vertex)))) return r;
+    if ((r = alloc(c->trk_type, N))) return r;
+    if ((r = alloc(c->trk_seq, n * sizeof(uint32_t)))) return r;
+    if ((r = alloc(c->trk_count, sizeof(unsigned long long)))) return r;
     HIP_TRY(hipMemsetAsync(c->trk_type.p, SSOLH_PATH_DROPPED, N, c->stream));
     HIP_TRY(hipMemsetAsync(c->trk_count.p, 0, sizeof(unsigned long long), c->stream));
     K.vtx = (ssolh_track_vertex*)c->trk_vtx.p;
@@ -2132,7 +2232,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   }
   // jobs
   if (job_suns_h) {
-    if ((r = dev_alloc(c->jobs, (njobs + 1) * sizeof(uint64_t) + njobs * sizeof(ssolh_sun)))) return r;
+    if ((r = alloc(c->jobs, (njobs + 1) * sizeof(uint64_t) + njobs * sizeof(ssolh_sun)))) return r;
     HIP_TRY(hipMemcpyAsync(c->jobs.p, job_first_h, (njobs + 1) * sizeof(uint64_t),
                            hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync((char*)c->jobs.p + (njobs + 1) * sizeof(uint64_t), job_suns_h,
@@ -2140,7 +2240,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   }
 
   // tallies (striped where many waves meet, see Tallies)
-  if ((r = dev_alloc(c->tallies, tally_bytes))) return r;
+  if ((r = alloc(c->tallies, tally_bytes))) return r;
   c->tally_bytes = tally_bytes;
   double* tb = (double*)c->tallies.p;
   c->T.global = tb;
@@ -2302,7 +2402,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     const uint32_t* order = nullptr;
     if (count && (chunked || first + count == N) && orderable) {
       const uint32_t nk = (uint32_t)nkeys, ni = c->ninstances;
-      if (int e = dev_alloc(c->start_hist, (size_t)nk * sizeof(uint32_t))) return e;
+      if (int e = alloc(c->start_hist, (size_t)nk * sizeof(uint32_t))) return e;
       uint32_t* hist = (uint32_t*)c->start_hist.p;
       uint2* key = (uint2*)c->hits.p;
       HIP_TRY(hipMemsetAsync(hist, 0, (size_t)nk * sizeof(uint32_t), c->stream));
@@ -2486,6 +2586,45 @@ SSOLH_API int ssolh_solve(ssolh_context* c, const struct ssolh_solve_args* a) {
   return run_solve(c, a->realisations, a->max_failed, a->rng_key, a->rng_counter,
                    a->precision == SSOLH_FP32, a->path_slots, npend, 1, nullptr, nullptr,
                    a->graph_iterations);
+}
+
+SSOLH_API int ssolh_solve_noalloc(ssolh_context* c, const struct ssolh_solve_args* a,
+                                  struct ssolh_tallies* out) {
+  if (!c || !a || !out || !a->realisations) {
+    set_err("ssolh_solve_noalloc: invalid arguments");
+    return SSOLH_ERR_ARG;
+  }
+  if (!c->has_scene) { set_err("ssolh_solve_noalloc: no scene uploaded"); return SSOLH_ERR_ARG; }
+  if (a->precision != SSOLH_FP64 && a->precision != SSOLH_FP32) {
+    set_err("ssolh_solve_noalloc: unknown precision %u", a->precision);
+    return SSOLH_ERR_ARG;
+  }
+  if (a->graph_iterations) {
+    set_err("ssolh_solve_noalloc: iterations are driven from the host (graph_iterations 0)");
+    return SSOLH_ERR_ARG;
+  }
+  if (!c->scratch) { set_err("ssolh_solve_noalloc: no scratch array on this device"); return SSOLH_ERR_MEM; }
+  HIP_TRY(hipSetDevice(c->device));
+  // The solve's buffers come from the scratch array: the context's own
+  // buffers are set aside for the call (nothing is freed or allocated) and
+  // put back after it.
+  DevBuf* bufs[] = {&c->state,        &c->rays,         &c->hits,        &c->lists[0],
+                    &c->lists[1],     &c->emitted,      &c->block_emits, &c->block_offsets,
+                    &c->scan_chunks,  &c->counters,     &c->tallies,     &c->start_hist};
+  constexpr size_t NB = sizeof(bufs) / sizeof(bufs[0]);
+  DevBuf saved[NB];
+  for (size_t i = 0; i < NB; ++i) {
+    saved[i] = *bufs[i];
+    *bufs[i] = DevBuf{};
+  }
+  Arena arena{(char*)c->scratch, SCRATCH_BYTES, 0};
+  const uint32_t npend = a->pending_entries ? a->pending_entries : 4;
+  int r = run_solve(c, a->realisations, a->max_failed, a->rng_key, a->rng_counter,
+                    a->precision == SSOLH_FP32, a->path_slots, npend, 1, nullptr, nullptr, 0,
+                    nullptr, nullptr, &arena);
+  if (!r) r = ssolh_tallies_download(c, out);
+  for (size_t i = 0; i < NB; ++i) *bufs[i] = saved[i];
+  return r;
 }
 
 SSOLH_API int ssolh_solve_batch(ssolh_context* c, const struct ssolh_batch_args* a,

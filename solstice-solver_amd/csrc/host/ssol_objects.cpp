@@ -149,6 +149,12 @@ res_T ssol_device_set_wavefront_size(struct ssol_device* d, const size_t slots) 
   return RES_OK;
 }
 
+res_T ssol_device_get_gpu_context(const struct ssol_device* d, void** ctx) {
+  if (!d || !ctx) return RES_BAD_ARG;
+  *ctx = d->gpu;
+  return d->gpu ? RES_OK : RES_BAD_OP;
+}
+
 // Spectrum (src/ssol_spectrum.c) -----------------------------------------------
 res_T ssol_spectrum_create(struct ssol_device* dev, struct ssol_spectrum** out) {
   if (!dev || !out) return RES_BAD_ARG;

@@ -185,6 +185,7 @@ def _declare(L):
         "ssol_device_ref_put": [VP],
         "ssol_device_set_precision": [VP, C.c_int],
         "ssol_device_set_wavefront_size": [VP, C.c_size_t],
+        "ssol_device_get_gpu_context": [VP, C.POINTER(VP)],
         "ssol_device_set_graph_iterations": [VP, C.c_uint],
         "ssol_device_autotune": [VP, VP, C.c_size_t, C.POINTER(AutotuneResult)],
         "ssol_solve_adjoint": [VP, VP, C.c_size_t, C.c_size_t, C.POINTER(VP),
