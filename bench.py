@@ -277,19 +277,27 @@ def cpu_model():
 
 def cpu_baseline(spec, seconds: float, threads: int):
     """The oracle (FP64 OpenMP restatement of the reference solver) on the
-    host cores, on a bounded sample of the same workload."""
+    host cores, on a bounded sample of the same workload: solves of 1e6
+    realisations and more (each solve has a fixed cost of thread start-up and
+    per-thread tally merge, which small solves would charge to the CPU), the
+    last one sized to end near `seconds`."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ctypes as O
     o = O.Oracle(spec, track_sampled_x_receiver=False)
-    chunk = 50_000
-    o.solve(2000, seed=99, nthreads=threads)  # warm-up (page-in, thread pool)
+    o.solve(20_000, seed=99, nthreads=threads)  # warm-up (page-in, thread pool)
     n = 0
-    t0 = time.perf_counter()
     k = 0
-    while time.perf_counter() - t0 < seconds:
+    chunk = 1_000_000
+    t0 = time.perf_counter()
+    while True:
         o.solve(chunk, seed=1000 + k, nthreads=threads)
         n += chunk
         k += 1
+        dt = time.perf_counter() - t0
+        left = seconds - dt
+        if left <= 0.05 * seconds:
+            break
+        chunk = max(1_000_000, int(n / dt * left))
     dt = time.perf_counter() - t0
     return n / dt / 1e6, n, dt, o
 

@@ -27,11 +27,16 @@ class Field:
     n: int = 4_000_000
     rng_seed: int = 1
     kw: dict = field(default_factory=dict)
+    path_slots: int = 0        # wavefront width of the GPU solves (0: default)
+    sample_only: int = -1      # >= 0: only this heliostat is sampled
 
     def scene(self, aim, slope):
         spec = S.config2(self.seed, n_helio=self.n_helio, radius=self.radius,
                          aim_perturb=np.asarray(aim), slope_errors=list(np.asarray(slope)),
                          rcv_size=self.rcv_size, rcv_cells=self.rcv_cells, **self.kw)
+        if self.sample_only >= 0:
+            for j, inst in enumerate(spec.instances[:-1]):
+                inst.sample = 1 if j == self.sample_only else 0
         axes = S.aim_axes(self.seed, self.n_helio, self.radius, self.kw.get("sun_dir"),
                           np.asarray(aim))
         return spec, axes
@@ -45,7 +50,7 @@ def receiver_of(spec):
 def forward(fld: Field, aim, slope, seed=None, n=None):
     """(absorbed, map) of the GPU solve."""
     spec, _ = fld.scene(aim, slope)
-    lib = L.Scene(spec)
+    lib = L.Scene(spec, path_slots=fld.path_slots)
     est = lib.solve(n or fld.n, seed=fld.rng_seed if seed is None else seed)
     absorbed = est.global_()["absorbed_by_receivers"][0]
     rcv, shape = receiver_of(spec)
@@ -57,7 +62,7 @@ def forward(fld: Field, aim, slope, seed=None, n=None):
 def adjoint(fld: Field, aim, slope, d_absorbed, d_map, seed=None, n=None):
     """Gradient [n_helio][3] (d/d a0, d/d a1, d/d slope error)."""
     spec, axes = fld.scene(aim, slope)
-    lib = L.Scene(spec)
+    lib = L.Scene(spec, path_slots=fld.path_slots)
     rcv, shape = receiver_of(spec)
     g = lib.solve_adjoint(n or fld.n, spec.instances[:-1], axes, d_absorbed=d_absorbed,
                           map_receiver=rcv if d_map is not None else None,
