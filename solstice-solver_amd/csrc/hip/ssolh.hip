@@ -155,16 +155,22 @@ enum : uint32_t {
   PF_HIT_RECEIVER = 1u << 4    // the path reached a receiver (path tracking)
 };
 
+// The FP64 form keeps per slot only what a bounce changes plus a few
+// records written once per path; the position and direction of the point a
+// ray leaves are those of the ray record (its origin is the last real point,
+// also past a virtual one).
 struct PathState {
-  double4* pos_wl;     // pos, wavelength
-  double4* dir_cos;    // direction, cos factor
-  double4* prev_init;  // previous outgoing weights (3), initial flux
-  double4* out_in0;    // outgoing weights of the last shaded point (3), its incoming flux
-  double4* nrm_t;      // normal of the current point, last hit distance
+  double2* cst;        // wavelength, cos factor (written once per path)
+  double4* pw;         // outgoing weights of the last real point (3); w: the
+                       // initial flux until the first shading, then the
+                       // incoming flux of the last virtual point crossed
+  double4* nrm;        // normal of the start point (written once per path)
   double4* pend;       // pending absorbed_by_receivers, extinguished, other
-  uint4* meta;         // path id lo/hi, rng block, flags
-  uint4* cur;          // (instance, shaded shape) index, prim, side, sampled slot
-  uint4* misc;         // depth, in medium | out medium << 16, survivor score, roulette interval
+  uint4* st;           // path id lo/hi, rng block, flags
+  uint4* sm;           // (instance, shaded shape) index, prim, depth | side << 30,
+                       // in medium | out medium << 16
+  uint4* aux;          // sampled slot, job (written once per path), survivor
+                       // score, roulette interval (rewritten past depth 64)
   uint32_t* pkey;      // [npend][slots]: receiver key (slot*2+side) or NONE
   double* pval;        // [npend][10][slots]
   uint32_t* qkey;      // [npend][slots]: per-primitive key or NONE
@@ -247,6 +253,7 @@ struct TrackParams {
   unsigned long long cap;
   uint8_t* type;             // [N] SSOLH_PATH_* of each realisation
   uint32_t* seq;             // [slots] vertices recorded by the slot's path
+  double4* pos;              // [slots] the point the slot's ray leaves
   double sun_len, inf_len;
 };
 
@@ -795,14 +802,14 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
     A.p[slot] = pp >= 0 ? (uint32_t)pp : NONE;
   }
 
-  P.pos_wl[slot] = make_double4(pos.x, pos.y, pos.z, wl);
-  P.dir_cos[slot] = make_double4(dir.x, dir.y, dir.z, cosf);
-  P.prev_init[slot] = make_double4(w0, w0, w0, w0);
-  P.nrm_t[slot] = make_double4(N.x, N.y, N.z, 0);
+  if (K.vtx) K.pos[slot] = make_double4(pos.x, pos.y, pos.z, 0);
+  P.cst[slot] = make_double2(wl, cosf);
+  P.pw[slot] = make_double4(w0, w0, w0, w0);
+  P.nrm[slot] = make_double4(N.x, N.y, N.z, 0);
   P.pend[slot] = make_double4(0, 0, 0, 0);
-  P.meta[slot] = make_uint4((uint32_t)path, (uint32_t)(path >> 32), rng.block, PF_ACTIVE);
-  P.cur[slot] = make_uint4(iss, st.prim, side, (uint32_t)inst.sampled);
-  P.misc[slot] = make_uint4(0, med | (med << 16), 0, 64u | (job << 8));
+  P.st[slot] = make_uint4((uint32_t)path, (uint32_t)(path >> 32), rng.block, PF_ACTIVE);
+  P.sm[slot] = make_uint4(iss, st.prim, 0u | (side << 30), med | (med << 16));
+  P.aux[slot] = make_uint4((uint32_t)inst.sampled, job, 0, 64u);
   for (uint32_t k = 0; k < prm.npend; ++k) {
     P.pkey[(size_t)k * prm.slots + slot] = NONE;
     P.qkey[(size_t)k * prm.slots + slot] = NONE;
@@ -923,45 +930,53 @@ __device__ __forceinline__ void shade_block(
   for (int i = 0; i < 6; ++i) fin.g[i] = 0;
   const uint32_t slot = live ? slot_of(list, q) : 0;
 
+  uint64_t path = 0;
   if (live) {
     const HitRec h = hits[slot];
-    uint4 meta = P.meta[slot];
-    uint4 cur = P.cur[slot];
-    uint4 misc = P.misc[slot];
-    double4 pw = P.pos_wl[slot];
-    double4 dc = P.dir_cos[slot];
-    double4 pv = P.prev_init[slot];
-    // outgoing weights of the last shaded point: none before the first one
-    double4 oi = (meta.w & PF_STAGE_BOUNCE) ? P.out_in0[slot] : make_double4(0, 0, 0, 0);
+    const uint4 st = P.st[slot];
+    const uint4 sm = P.sm[slot];
+    const double2 cw = P.cst[slot];
+    // outgoing weights of the last real point (3); w: the initial flux
+    // before the first shading, then the incoming flux of the last virtual
+    // point the path crossed
+    double4 pv = P.pw[slot];
     double4 pd = P.pend[slot];
-    const uint64_t path = (uint64_t)meta.x | ((uint64_t)meta.y << 32);
+    // the ray of the emitting point: its origin (the last real point, also
+    // past a virtual point, whose ray keeps that origin) and direction
+    const RayRec& rr = rays[slot];
+    const d3 rorg0 = mk(rr.ox, rr.oy, rr.oz), rdir0 = mk(rr.dx, rr.dy, rr.dz);
+    path = (uint64_t)st.x | ((uint64_t)st.y << 32);
     PathRng rng;
-    rng.init(prm.key, prm.counter + path, meta.z);
-    uint32_t flags = meta.w;
-    fin.samp_slot = (int)cur.w;
-
-    uint32_t in_med = misc.y & 0xFFFFu, out_med = misc.y >> 16;
-    uint32_t depth = misc.x, score = misc.z, rinterval = misc.w & 0xFFu;
-    fin.job = misc.w >> 8;
-    d3 pos = mk(pw.x, pw.y, pw.z);
-    const double wl = pw.w;
-    d3 dir = mk(dc.x, dc.y, dc.z);
+    rng.init(prm.key, prm.counter + path, st.z);
+    const uint32_t flags0 = st.w;
+    uint32_t flags = flags0;
+    uint32_t iss = sm.x, prim = sm.y;
+    uint32_t depth = sm.z & 0x3FFFFFFFu, side = sm.z >> 30;
+    uint32_t in_med = sm.w & 0xFFFFu, out_med = sm.w >> 16;
+    // sampled slot, job, survivor score and roulette interval: read at the
+    // end of the path and past the first roulette depth only
+    uint4 aux = make_uint4(0, 0, 0, 64);
+    bool have_aux = false;
+    auto load_aux = [&]() {
+      if (!have_aux) { aux = P.aux[slot]; have_aux = true; }
+    };
+    const double wl = cw.x;
+    d3 pos = rorg0;
+    d3 dir = (flags & PF_STAGE_BOUNCE) ? rdir0 : -rdir0;  // the shadow ray points at the sun
     d3 N = mk(0, 0, 0);
     if (!(flags & PF_STAGE_BOUNCE)) {
       // normal of the start point (later points take theirs from the hit)
-      const double4 nt = P.nrm_t[slot];
+      const double4 nt = P.nrm[slot];
       N = mk(nt.x, nt.y, nt.z);
     }
     double hit_t = 0;
     d3 rorg = pos, rdir = dir;  // origin/direction of the ray of the point
     bool head = false;
-    uint32_t iss = cur.x, prim = cur.y, side = cur.z;
 
     if (!(flags & PF_STAGE_BOUNCE)) {
       // result of the shadow ray
       if (h.iss != NONE) {
         fin.kind = FIN_DONE;
-        fin.g[0] = dc.w;      // cos factor
         fin.g[2] = pv.w;      // shadowed = initial flux
       } else {
         head = true;
@@ -972,45 +987,47 @@ __device__ __forceinline__ void shade_block(
       const bool found = h.iss != NONE;
       const bool hit_virtual = (flags & PF_HIT_VIRTUAL) != 0;
       const bool in_atm = (flags & PF_IN_ATM) != 0;
-      bool last = false;
+      // flux that left the emitting point
+      const double left = hit_virtual ? pv.w : pv.x;
       if (!found) {
-        last = true;
         // escape point (src/ssol_solver.c:1012-1019)
-        if (K.vtx && K.inf_len > 0)
-          track_vertex(K, slot, path, pos + dir * K.inf_len, oi.x);
-        if (out_med != S.air) fin.kind = FIN_FAILED;  // ray lost outside air
-      }
-      if (fin.kind == FIN_NONE && (last || !hit_virtual)) {
-        const double absorbed = pv.x - oi.w;
-        if (in_atm) pd.y += absorbed; else pd.z += absorbed;
-        if (last) {
-          fin.kind = FIN_DONE;
-          fin.g[3] = oi.x;  // missing
+        if (K.vtx && K.inf_len > 0) {
+          const double4 kp = K.pos[slot];
+          track_vertex(K, slot, path, mk(kp.x, kp.y, kp.z) + dir * K.inf_len, left);
+        }
+        if (out_med != S.air) {
+          fin.kind = FIN_FAILED;  // ray lost outside air
         } else {
-          pv.x = oi.x; pv.y = oi.y; pv.z = oi.z;
+          // past a virtual point the segment from the last real point
+          // ends at the virtual one (a real point's segment was counted
+          // at that point)
+          if (hit_virtual) {
+            const double absorbed = pv.x - pv.w;
+            if (in_atm) pd.y += absorbed; else pd.z += absorbed;
+          }
+          fin.kind = FIN_DONE;
+          fin.g[3] = left;  // missing
         }
       }
       if (fin.kind == FIN_NONE) {
         depth += hit_virtual ? 0u : 1u;
-        if (depth > rinterval && depth % rinterval == 1) {
-          if (rng.next() > 0.5) {
-            score += 1;
-            rinterval = 16;
-          } else {
-            fin.kind = FIN_KILLED;
+        if (depth > 64) {
+          load_aux();
+          const uint32_t rinterval = aux.w;
+          if (depth > rinterval && depth % rinterval == 1) {
+            if (rng.next() > 0.5) {
+              aux.z += 1;
+              aux.w = 16;
+              P.aux[slot] = aux;
+            } else {
+              fin.kind = FIN_KILLED;
+            }
           }
         }
       }
       if (fin.kind == FIN_NONE) {
         // point_update_from_hit (src/ssol_solver.c:360-411)
-        // the ray left the current point along dir, except that a ray past
-        // a virtual point keeps the origin of the ray that reached it
-        if (hit_virtual) {
-          const RayRec& rr = rays[slot];
-          rorg = mk(rr.ox, rr.oy, rr.oz);
-        } else {
-          rorg = pos;
-        }
+        rorg = rorg0;
         rdir = dir;
         iss = h.iss;
         prim = h.prim;
@@ -1022,7 +1039,7 @@ __device__ __forceinline__ void shade_block(
         in_med = out_med;
         hit_t = h.t;
         head = true;
-        if (K.vtx) track_vertex(K, slot, path, pos, oi.x);  // src/ssol_solver.c:1080-1083
+        if (K.vtx) track_vertex(K, slot, path, pos, left);  // src/ssol_solver.c:1080-1083
       }
     }
 
@@ -1106,8 +1123,9 @@ __device__ __forceinline__ void shade_block(
           w10[8] = (in2 - in0) * kabs; w10[9] = (in1 - in0) * kabs;
           const uint32_t rkey = (uint32_t)is.receiver * 2u + (side == SSOL_FRONT_ ? 0u : 1u);
           if (!pend_add(P.pkey + slot, P.pval + slot, stride, prm.npend, rkey, w10)) {
+            load_aux();
             spill(T.rcv + (size_t)(blockIdx.x & T.mask_r) * T.nrcv * 40u, rkey, w10, &C->spills);
-            spill(T.sxr + (size_t)(blockIdx.x & T.mask_x) * T.nsamp * T.nrcv * 40u, (uint64_t)cur.w * S.nrcv * 2u + rkey, w10, &C->spills);
+            spill(T.sxr + (size_t)(blockIdx.x & T.mask_x) * T.nsamp * T.nrcv * 40u, (uint64_t)aux.x * S.nrcv * 2u + rkey, w10, &C->spills);
           }
           if (is.prim_tally >= 0) {
             const uint32_t pkey = (uint32_t)(((uint64_t)is.prim_tally + prim) * 2u +
@@ -1120,10 +1138,15 @@ __device__ __forceinline__ void shade_block(
         } else {
           pd.z += in0 * kabs;
         }
-        if (out0 == 0) {
-          // weight is zero: the segment ends here
+        // extinction of the segment that reached this point, counted here at
+        // a real point (past virtual points the segment runs from the last
+        // real point); a virtual point counts it only if the path ends
+        if (!virt || out0 == 0) {
           const double absorbed = pv.x - in0;
           if (in_atm) pd.y += absorbed; else pd.z += absorbed;
+        }
+        if (out0 == 0) {
+          // weight is zero: the segment ends here
           fin.kind = FIN_DONE;
           fin.g[3] = 0;
         } else {
@@ -1132,48 +1155,51 @@ __device__ __forceinline__ void shade_block(
             e.r.ox = rorg.x; e.r.oy = rorg.y; e.r.oz = rorg.z;
             e.r.dx = rdir.x; e.r.dy = rdir.y; e.r.dz = rdir.z;
             e.r.tmin = nextafterf((float)hit_t, FLT_MAX);
+            pv.w = in0;
           } else {
             e.r.ox = pos.x; e.r.oy = pos.y; e.r.oz = pos.z;
             e.r.dx = dir.x; e.r.dy = dir.y; e.r.dz = dir.z;
             e.r.tmin = 0;
+            pv = make_double4(out0, out1, out2, pv.w);
           }
           e.r.iss_from = iss;
           e.r.prim_from = prim;
           e.r.flags = side << RF_SIDE_SHIFT;
           flags = (flags & PF_HIT_RECEIVER) | PF_ACTIVE | PF_STAGE_BOUNCE |
                   (virt ? PF_HIT_VIRTUAL : 0u) | (in_atm ? PF_IN_ATM : 0u);
-          oi = make_double4(out0, out1, out2, in0);
           out_med = next_med;
         }
       }
     }
 
+    if (fin.kind != FIN_NONE) {
+      load_aux();
+      fin.samp_slot = (int)aux.x;
+      fin.job = aux.y;
+    }
     if (K.vtx && fin.kind != FIN_NONE) {
       // path type (src/ssol_solver.c:931, 1092-1094, 1111-1125)
       uint8_t ty;
       if (fin.kind == FIN_KILLED) ty = SSOLH_PATH_DROPPED;
       else if (fin.kind == FIN_FAILED) ty = SSOLH_PATH_ERROR;
-      else if (!(meta.w & PF_STAGE_BOUNCE) && h.iss != NONE) ty = SSOLH_PATH_SHADOW;
+      else if (!(flags0 & PF_STAGE_BOUNCE) && h.iss != NONE) ty = SSOLH_PATH_SHADOW;
       else ty = (flags & PF_HIT_RECEIVER) ? SSOLH_PATH_SUCCESS : SSOLH_PATH_MISSING;
       K.type[path] = ty;
     }
     if (fin.kind == FIN_DONE) {
-      fin.g[0] = dc.w;
+      fin.g[0] = cw.y;  // cos factor
       fin.g[1] = pd.x;
       fin.g[4] = pd.y;
       fin.g[5] = pd.z;
-      fin.factor = (double)(1u << score);
+      fin.factor = (double)(1u << aux.z);
       for (int i = 0; i < 6; ++i) fin.g[i] *= fin.factor;
     }
     if (fin.kind == FIN_NONE) {
-      P.pos_wl[slot] = make_double4(pos.x, pos.y, pos.z, wl);
-      P.dir_cos[slot] = make_double4(dir.x, dir.y, dir.z, dc.w);
-      P.prev_init[slot] = pv;
-      P.out_in0[slot] = oi;
+      P.pw[slot] = pv;
       P.pend[slot] = pd;
-      P.meta[slot] = make_uint4(meta.x, meta.y, rng.block, flags);
-      P.cur[slot] = make_uint4(iss, prim, side, cur.w);
-      P.misc[slot] = make_uint4(depth, in_med | (out_med << 16), score, rinterval | (fin.job << 8));
+      P.st[slot] = make_uint4(st.x, st.y, rng.block, flags);
+      P.sm[slot] = make_uint4(iss, prim, depth | (side << 30), in_med | (out_med << 16));
+      if (K.vtx) K.pos[slot] = make_double4(pos.x, pos.y, pos.z, 0);
     }
   }
 
@@ -1267,14 +1293,12 @@ __device__ __forceinline__ void shade_block(
 
   // --- refill finished slots with their next realisation -------------------
   const bool want = live && fin.kind != FIN_NONE && !C->aborted;
-  const uint64_t id = want ? (((uint64_t)P.meta[slot].y << 32) | P.meta[slot].x) + prm.slots : 0;
+  const uint64_t id = want ? path + prm.slots : 0;
   int samp_slot = 0;
   bool gen = false;
   if (REFILL && want && id < prm.N) {
     generate(S, P, prm, slot, id, e, samp_slot, A, K);
     gen = true;
-  } else if (live && fin.kind != FIN_NONE) {
-    P.meta[slot] = make_uint4(0, 0, 0, 0);
   }
   flush_sampled_count(gen, samp_slot, T);
 
@@ -1659,7 +1683,7 @@ struct ssolh_context {
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, scan_chunks, counters,
       tallies, jobs;
   DevBuf adj_state, adj_in, adj_grad;
-  DevBuf trk_vtx, trk_type, trk_seq, trk_count;
+  DevBuf trk_vtx, trk_type, trk_seq, trk_count, trk_pos;
   uint64_t trk_paths = 0;           // realisations of the last tracked solve
   unsigned long long trk_nvtx = 0;  // vertices it produced
   uint64_t trk_cap = 0;
@@ -1884,7 +1908,7 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
                     &c->block_emits, &c->block_offsets, &c->scan_chunks, &c->counters, &c->tallies,
                     &c->jobs, &c->start_hist,
                     &c->adj_state, &c->adj_in, &c->adj_grad, &c->trk_vtx, &c->trk_type,
-                    &c->trk_seq, &c->trk_count})
+                    &c->trk_seq, &c->trk_count, &c->trk_pos})
     if (b->p) (void)hipFree(b->p);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -2091,11 +2115,12 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   // first reflections, ...), which keeps the waves coherent; larger solves
   // run as many as fit, each slot taking its next realisation in place.
   uint64_t slots64 = path_slots;
-  const size_t per_slot = 6 * 4 * (fp32 ? sizeof(float) : sizeof(double)) + 3 * sizeof(uint4) +
+  const size_t per_slot = (fp32 ? 6 * sizeof(float4) + 3 * sizeof(uint4)
+                               : sizeof(double2) + 3 * sizeof(double4) + 3 * sizeof(uint4)) +
                           (size_t)npend * 2 * (sizeof(uint32_t) + 10 * (fp32 ? sizeof(float) : sizeof(double))) +
                           (fp32 ? sizeof(RayRec32) + sizeof(HitRec32) : sizeof(RayRec) + sizeof(HitRec)) +
                           2 * sizeof(uint32_t) + 1 + (adj ? sizeof(double4) + sizeof(uint32_t) : 0) +
-                          (trk ? sizeof(uint32_t) : 0);
+                          (trk ? sizeof(uint32_t) + sizeof(double4) : 0);
   if (!slots64 && arena) {
     // from the scratch array: what is left beside the tallies, the
     // per-block scan arrays, counters and start-instance histogram
@@ -2128,49 +2153,55 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   const uint32_t slots = (uint32_t)slots64;
   const uint32_t max_blocks = slots / BLOCK;
 
-  // path state: 9 arrays of 16- or 32-byte records and the pending entries,
+  // path state: arrays of 16- or 32-byte records and the pending entries,
   // each array starting on a 256-byte boundary
   const size_t n = slots;
   const size_t real = fp32 ? sizeof(float) : sizeof(double);
-  const size_t sizes[13] = {n * 4 * real, n * 4 * real, n * 4 * real, n * 4 * real,
-                            n * 4 * real, n * 4 * real,
-                            n * sizeof(uint4), n * sizeof(uint4), n * sizeof(uint4),
-                            npend * n * sizeof(uint32_t), npend * n * sizeof(uint32_t),
-                            npend * n * 10 * real, npend * n * 10 * real};
+  const size_t pend_keys = npend * n * sizeof(uint32_t), pend_vals = npend * n * 10 * real;
+  std::vector<size_t> sizes;
+  if (fp32)
+    sizes = {n * sizeof(float4), n * sizeof(float4), n * sizeof(float4), n * sizeof(float4),
+             n * sizeof(float4), n * sizeof(float4), n * sizeof(uint4), n * sizeof(uint4),
+             n * sizeof(uint4), pend_keys, pend_keys, pend_vals, pend_vals};
+  else
+    sizes = {n * sizeof(double2), n * sizeof(double4), n * sizeof(double4), n * sizeof(double4),
+             n * sizeof(uint4), n * sizeof(uint4), n * sizeof(uint4), pend_keys, pend_keys,
+             pend_vals, pend_vals};
   size_t bytes_state = 0;
   for (size_t sz : sizes) bytes_state += (sz + 255) / 256 * 256;
   int r = alloc(c->state, bytes_state);
   if (r) return r;
   {
     char* p = (char*)c->state.p;
-    void* arr[13];
-    for (int i = 0; i < 13; ++i) { arr[i] = p; p += (sizes[i] + 255) / 256 * 256; }
-    c->P.pos_wl = (double4*)arr[0];
-    c->P.dir_cos = (double4*)arr[1];
-    c->P.prev_init = (double4*)arr[2];
-    c->P.out_in0 = (double4*)arr[3];
-    c->P.nrm_t = (double4*)arr[4];
-    c->P.pend = (double4*)arr[5];
-    c->P.meta = (uint4*)arr[6];
-    c->P.cur = (uint4*)arr[7];
-    c->P.misc = (uint4*)arr[8];
-    c->P.pkey = (uint32_t*)arr[9];
-    c->P.qkey = (uint32_t*)arr[10];
-    c->P.pval = (double*)arr[11];
-    c->P.qval = (double*)arr[12];
-    c->P32.pos = (float4*)arr[0];
-    c->P32.dir_wl = (float4*)arr[1];
-    c->P32.prev_init = (float4*)arr[2];
-    c->P32.out_in0 = (float4*)arr[3];
-    c->P32.nrm_t = (float4*)arr[4];
-    c->P32.pend_cos = (float4*)arr[5];
-    c->P32.meta = (uint4*)arr[6];
-    c->P32.cur = (uint4*)arr[7];
-    c->P32.misc = (uint4*)arr[8];
-    c->P32.pkey = (uint32_t*)arr[9];
-    c->P32.qkey = (uint32_t*)arr[10];
-    c->P32.pval = (float*)arr[11];
-    c->P32.qval = (float*)arr[12];
+    std::vector<void*> arr(sizes.size());
+    for (size_t i = 0; i < sizes.size(); ++i) { arr[i] = p; p += (sizes[i] + 255) / 256 * 256; }
+    if (fp32) {
+      c->P32.pos = (float4*)arr[0];
+      c->P32.dir_wl = (float4*)arr[1];
+      c->P32.prev_init = (float4*)arr[2];
+      c->P32.out_in0 = (float4*)arr[3];
+      c->P32.nrm_t = (float4*)arr[4];
+      c->P32.pend_cos = (float4*)arr[5];
+      c->P32.meta = (uint4*)arr[6];
+      c->P32.cur = (uint4*)arr[7];
+      c->P32.misc = (uint4*)arr[8];
+      c->P32.pkey = (uint32_t*)arr[9];
+      c->P32.qkey = (uint32_t*)arr[10];
+      c->P32.pval = (float*)arr[11];
+      c->P32.qval = (float*)arr[12];
+    } else {
+      c->P.cst = (double2*)arr[0];
+      c->P.pw = (double4*)arr[1];
+      c->P.nrm = (double4*)arr[2];
+      c->P.pend = (double4*)arr[3];
+      c->P.st = (uint4*)arr[4];
+      c->P.sm = (uint4*)arr[5];
+      c->P.aux = (uint4*)arr[6];
+      c->P.pkey = (uint32_t*)arr[7];
+      c->P.qkey = (uint32_t*)arr[8];
+      c->P.pval = (double*)arr[9];
+      c->P.qval = (double*)arr[10];
+    }
   }
   if ((r = alloc(c->rays, n * (fp32 ? sizeof(RayRec32) : sizeof(RayRec))))) return r;
   if ((r = alloc(c->hits, n * (fp32 ? sizeof(HitRec32) : sizeof(HitRec))))) return r;
@@ -2217,6 +2248,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     if ((r = alloc(c->trk_vtx, cap * sizeof(ssolh_track_vertex)))) return r;
     if ((r = alloc(c->trk_type, N))) return r;
     if ((r = alloc(c->trk_seq, n * sizeof(uint32_t)))) return r;
+    if ((r = alloc(c->trk_pos, n * sizeof(double4)))) return r;
     if ((r = alloc(c->trk_count, sizeof(unsigned long long)))) return r;
     HIP_TRY(hipMemsetAsync(c->trk_type.p, SSOLH_PATH_DROPPED, N, c->stream));
     HIP_TRY(hipMemsetAsync(c->trk_count.p, 0, sizeof(unsigned long long), c->stream));
@@ -2225,6 +2257,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     K.cap = cap;
     K.type = (uint8_t*)c->trk_type.p;
     K.seq = (uint32_t*)c->trk_seq.p;
+    K.pos = (double4*)c->trk_pos.p;
     K.sun_len = trk->sun_len;
     K.inf_len = trk->inf_len;
     c->trk_cap = cap;
