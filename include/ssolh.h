@@ -230,8 +230,9 @@ struct ssolh_scene {
   uint32_t nnodes;
   const struct ssolh_bvh_node* nodes;   /* BLASes then TLAS */
   uint32_t tlas_root;
-  uint32_t stack_depth;       /* traversal stack entries a ray can need:
-                                 TLAS depth + deepest BLAS + 2 (<= 48) */
+  uint32_t stack_depth;       /* TLAS depth + deepest BLAS + 2 of the BVH2
+                                 (informational: the upload derives the
+                                 stack bound of its 4-wide nodes) */
 
   uint32_t ntriangles;
   const struct ssolh_triangle* triangles;
@@ -448,6 +449,13 @@ SSOLH_API int ssolh_test_threefry
  * w*64+63] for every lane i, nwaves waves of 64 lanes */
 SSOLH_API int ssolh_test_wave_sum
   (ssolh_context* ctx, uint32_t nwaves, const double* in, double* out);
+
+/* Traversal stack of the trace kernels: caps the entries each lane keeps in
+ * LDS (16, 24 or 32; 0 restores the default, the smallest that holds the
+ * scene's bound) for the solves that follow, so that the rest of the bound
+ * goes to the device-memory overflow; *bound (if not NULL) = the uploaded
+ * scene's bound (0 without a scene). The results do not depend on the cap. */
+SSOLH_API int ssolh_test_trace_stack(ssolh_context* ctx, uint32_t lds_cap, uint32_t* bound);
 
 /* Bandwidth of a plain device copy of `bytes` bytes repeated `reps` times
  * (read + write bytes over the HIP-event time), in GB/s. Calibrates the HBM

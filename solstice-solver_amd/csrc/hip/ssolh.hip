@@ -35,6 +35,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -48,13 +49,14 @@ namespace {
 
 constexpr int BLOCK = 128;           // threads per block (2 waves)
 constexpr int WAVES = BLOCK / 64;
-constexpr int MAX_STACK = 48;        // traversal stack entries per thread (LDS), largest kernel
+constexpr int MAX_STACK = 32;        // traversal stack entries per thread in LDS, largest kernel
 constexpr int SCAN_THREADS = 1024;   // threads of a scan block
 constexpr uint32_t SENTINEL = 0xFFFFFFFFu;
 constexpr uint32_t LEAF = 0x80000000u;
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t TOP_REF = 0x40000000u;  // inner-node reference into the LDS copy
-constexpr int TOP_NODES = 63;              // TLAS nodes staged in LDS (4 KB): 6 levels
+constexpr uint32_t EMPTY4 = 0xFFFFFFFEu;   // absent child of a 4-wide node
+constexpr int TOP_NODES = 31;              // TLAS nodes staged in LDS (3.9 KB): 3 levels
 constexpr int MAX_PEND = 8;          // compile-time bound of pending entries
 
 // Per-thread message of the last failing call
@@ -93,6 +95,20 @@ struct alignas(8) IssInfo {
   int64_t prim_tally;      // first per-primitive slot or -1
 };
 
+// BVH node of the traversal: four child boxes (by axis, then child) and
+// four child references in the form the traversal follows (inner node
+// index, TOP_REF | index into the LDS copy, LEAF | (count - 1) << 28 | first
+// item, or EMPTY4). Built at upload by collapsing the host's BVH2 (each
+// node takes the children and grandchildren of a BVH2 node, opening the
+// largest child boxes first), which halves the depth a ray descends.
+struct alignas(16) Node4 {
+  float lo[3][4];
+  float hi[3][4];
+  uint32_t child[4];
+  uint32_t pad[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 layout");
+
 struct DScene {
   Spectra spectra;
   const ssolh_medium* media;
@@ -100,11 +116,11 @@ struct DScene {
   const ssolh_material* mtls;
   const ssolh_shape* shapes;
   const ssolh_object* objects;
-  const ssolh_bvh_node* nodes;
+  const Node4* nodes;
   uint32_t tlas_root;
   // copies of the TLAS's top levels (breadth first, children in the set
   // referenced as TOP_REF | index), staged in LDS by the trace kernels
-  const ssolh_bvh_node* top;
+  const Node4* top;
   uint32_t ntop;
   uint32_t root_ref;         // first node reference of a traversal
   const ssolh_triangle* tris;
@@ -528,24 +544,106 @@ __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t is
 // Node i of the LDS copy of the top levels, read through an LDS (address
 // space 3) pointer: the load is a ds_read, where a pointer that may address
 // either LDS or global memory would make it a flat access
-__device__ __forceinline__ ssolh_bvh_node lds_node(const ssolh_bvh_node* lds, uint32_t i) {
-  typedef const __attribute__((address_space(3))) uint32_t* lds_u32;
-  const lds_u32 p = (lds_u32)(lds + i);
-  ssolh_bvh_node n;
-  uint32_t* w = (uint32_t*)&n;
-  for (int k = 0; k < (int)(sizeof(n) / 4); ++k) w[k] = p[k];
+__device__ __forceinline__ Node4 lds_node4(const Node4* lds, uint32_t i) {
+  typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(3))) v4u* lds_u4;
+  const lds_u4 p = (lds_u4)(lds + i);
+  Node4 n;
+  v4u* w = (v4u*)&n;
+  for (int k = 0; k < (int)(sizeof(n) / 16); ++k) w[k] = p[k];
   return n;
 }
 
-// STACK: LDS stack entries per lane, at least the scene's stack_depth (the
-// smaller the stack, the more blocks share a CU). Four waves per SIMD caps
-// the kernel at 128 VGPRs; the few values that spill are cold.
+// Child c's box of a 4-wide node (the arithmetic of box_hit)
+__device__ __forceinline__ bool box_hit4(const FRay& r, const Node4& n, int c, float tmin,
+                                         float tmax, float& tnear) {
+  const float lo[3] = {n.lo[0][c], n.lo[1][c], n.lo[2][c]};
+  const float hi[3] = {n.hi[0][c], n.hi[1][c], n.hi[2][c]};
+  return box_hit(r, lo, hi, tmin, tmax, tnear);
+}
+
+// Traversal stack of one lane: its first STACK entries in LDS (stride
+// BLOCK), the rest, when the scene's bound exceeds STACK, in a global array
+// of `extra` entries per work-list position (stride: the list's capacity)
+struct TStack {
+  uint32_t* lds;
+  uint32_t* ovf;   // this lane's first overflow entry, or null
+  uint32_t stride, extra;
+  int sp;
+};
+
+template <int STACK>
+__device__ __forceinline__ void tpush(TStack& s, uint32_t v) {
+  if (s.sp < STACK) s.lds[s.sp * BLOCK] = v;
+  else if ((uint32_t)(s.sp - STACK) < s.extra) s.ovf[(size_t)(s.sp - STACK) * s.stride] = v;
+  ++s.sp;
+}
+
+// false: the stack is empty (or, never with the host's bound, past its end)
+template <int STACK>
+__device__ __forceinline__ bool tpop(TStack& s, uint32_t& ref) {
+  if (s.sp == 0) return false;
+  --s.sp;
+  if (s.sp < STACK) {
+    ref = s.lds[s.sp * BLOCK];
+  } else {
+    if ((uint32_t)(s.sp - STACK) >= s.extra) return false;
+    ref = s.ovf[(size_t)(s.sp - STACK) * s.stride];
+  }
+  return true;
+}
+
+// One visit of a 4-wide inner node: the children whose boxes the ray
+// crosses in [tmin, tmax], nearest first; the nearest becomes `ref`, the
+// others are pushed farthest first (so the next nearest is on top). With no
+// child crossed, `ref` is popped; false when the stack is empty.
+template <int STACK>
+__device__ __forceinline__ bool visit4(const Node4& n, const FRay& fr, float tmin, float tmax,
+                                       uint32_t& ref, TStack& st) {
+  float k[4];
+  uint32_t r[4];
+  for (int c = 0; c < 4; ++c) {
+    float tn;
+    const bool h = box_hit4(fr, n, c, tmin, tmax, tn) & (n.child[c] != EMPTY4);
+    k[c] = h ? tn : INFINITY;
+    r[c] = n.child[c];
+  }
+  // sorting network (0,1)(2,3)(0,2)(1,3)(1,2)
+  auto cs = [&](int a, int b) {
+    const bool sw = k[b] < k[a];
+    const float ka = k[a], kb = k[b];
+    const uint32_t ra = r[a], rb = r[b];
+    k[a] = sw ? kb : ka; k[b] = sw ? ka : kb;
+    r[a] = sw ? rb : ra; r[b] = sw ? ra : rb;
+  };
+  cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
+  const int cnt = (k[0] < INFINITY) + (k[1] < INFINITY) + (k[2] < INFINITY) + (k[3] < INFINITY);
+  if (cnt == 0) return tpop<STACK>(st, ref);
+  if (cnt >= 4) tpush<STACK>(st, r[3]);
+  if (cnt >= 3) tpush<STACK>(st, r[2]);
+  if (cnt >= 2) tpush<STACK>(st, r[1]);
+  ref = r[0];
+  return true;
+}
+
+// Overflow of the traversal stacks past their LDS part: `extra` entries per
+// work-list position, `stride` = the list's capacity (null when the scene's
+// stack bound fits in LDS)
+struct TOvf {
+  uint32_t* base;
+  uint32_t stride, extra;
+};
+
+// STACK: LDS stack entries per lane (the smaller the stack, the more blocks
+// share a CU); the scene's bound beyond them goes to the overflow array. Four
+// waves per SIMD caps the kernel at 128 VGPRs; the few values that spill are
+// cold.
 // Entry q of the work list: traversal, filter, hit record
 template <int STACK>
 __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __restrict__ list,
                                           uint32_t q, const RayRec* __restrict__ rays,
                                           HitRec* __restrict__ hits, uint32_t* lds_stack,
-                                          const ssolh_bvh_node* lds_top) {
+                                          const Node4* lds_top, const TOvf& ovf) {
 
   const uint32_t slot = slot_of(list, q);
   const RayRec r = rays[slot];
@@ -560,8 +658,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
   ts.prim = 0;
   ts.punched = false;
 
-  uint32_t* stack = lds_stack + threadIdx.x;
-  int sp = 0;
+  TStack st{lds_stack + threadIdx.x, ovf.base ? ovf.base + q : nullptr, ovf.stride, ovf.extra, 0};
   FRay fr = make_fray(ow, dw);
   d3 o = ow, d = dw;        // current-level ray (double)
   Shear sh = make_shear(dw);
@@ -582,39 +679,14 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
   // SENTINEL (all bits set) reads as a leaf reference and ends the descent.
   uint32_t ref = S.root_ref;
   bool alive = true;
-  auto pop = [&]() -> bool {
-    if (sp == 0) return false;
-    --sp;
-    ref = stack[sp * BLOCK];
-    return true;
-  };
   // box culling distance: the best hit so far, widened by 2 ulp (FP32)
   float tmax = FLT_MAX;
   while (alive) {
     while (!(ref & LEAF)) {
       // the whole node in one round of loads (LDS or global memory, each
-      // through its own address space), then both boxes
-      ssolh_bvh_node n;
-      if (ref & TOP_REF)
-        n = lds_node(lds_top, ref & 0x0FFFFFFFu);
-      else
-        n = S.nodes[ref];
-      float tn0, tn1;
-      const bool b0 = box_hit(fr, n.lo[0], n.hi[0], ftmin, tmax, tn0);
-      const bool b1 = box_hit(fr, n.lo[1], n.hi[1], ftmin, tmax, tn1);
-      const bool h0 = b0 & (n.count[0] != SSOLH_BVH_EMPTY);
-      const bool h1 = b1 & (n.count[1] != SSOLH_BVH_EMPTY);
-      // child references come encoded (leaf bit, count, first item)
-      if (h0 && h1) {
-        const bool near0 = tn0 <= tn1;
-        const uint32_t rn = near0 ? n.child[0] : n.child[1], rf = near0 ? n.child[1] : n.child[0];
-        if (sp < STACK) { stack[sp * BLOCK] = rf; ++sp; }
-        ref = rn;
-      } else if (h0) {
-        ref = n.child[0];
-      } else if (h1) {
-        ref = n.child[1];
-      } else if (!pop()) {
+      // through its own address space), then its four boxes
+      const Node4 n = (ref & TOP_REF) ? lds_node4(lds_top, ref & 0x0FFFFFFFu) : S.nodes[ref];
+      if (!visit4<STACK>(n, fr, ftmin, tmax, ref, st)) {
         alive = false;
         break;
       }
@@ -626,7 +698,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
       fr = make_fray(ow, dw);
       o = ow;
       d = dw;
-      alive = pop();
+      alive = tpop<STACK>(st, ref);
       continue;
     }
     const uint32_t first = ref & 0x0FFFFFFFu;
@@ -634,7 +706,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
       // TLAS leaf: one instance, enter its BLAS
       const uint32_t ii = S.tlas_inst[first];
       if (ii == skip_inst) {
-        alive = pop();
+        alive = tpop<STACK>(st, ref);
         continue;
       }
       const ssolh_instance& inst = S.insts[ii];
@@ -645,7 +717,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
       sh = make_shear(d);
       iss_first = inst.isshape_first;
       in_blas = true;
-      if (sp < STACK) { stack[sp * BLOCK] = SENTINEL; ++sp; }
+      tpush<STACK>(st, SENTINEL);
       ref = S.objects[inst.object].blas_root;
       continue;
     }
@@ -668,7 +740,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
       if (any_hit) { done = true; break; }
     }
     if (done) break;
-    alive = pop();
+    alive = tpop<STACK>(st, ref);
   }
 
   HitRec h;
@@ -691,9 +763,9 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
 template <int STACK, bool LOOP>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict__ count_ptr,
-        const RayRec* __restrict__ rays, HitRec* __restrict__ hits, unsigned* fetch) {
+        const RayRec* __restrict__ rays, HitRec* __restrict__ hits, unsigned* fetch, TOvf ovf) {
   __shared__ uint32_t lds_stack[STACK * BLOCK];
-  __shared__ ssolh_bvh_node lds_top[TOP_NODES];
+  __shared__ Node4 lds_top[TOP_NODES];
   const uint32_t count = *count_ptr;
   if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: the staging syncs
   for (uint32_t i = threadIdx.x; i < S.ntop; i += BLOCK) lds_top[i] = S.top[i];
@@ -701,11 +773,11 @@ k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict_
   if (LOOP) {
     for_each_block((count + BLOCK - 1) / BLOCK, fetch, [&](uint32_t b) {
       const uint32_t q = b * BLOCK + threadIdx.x;
-      if (q < count) trace_ray<STACK>(S, list, q, rays, hits, lds_stack, lds_top);
+      if (q < count) trace_ray<STACK>(S, list, q, rays, hits, lds_stack, lds_top, ovf);
     });
   } else {
     const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-    if (q < count) trace_ray<STACK>(S, list, q, rays, hits, lds_stack, lds_top);
+    if (q < count) trace_ray<STACK>(S, list, q, rays, hits, lds_stack, lds_top, ovf);
   }
 }
 
@@ -1677,11 +1749,12 @@ struct ssolh_context {
   bool has_scene = false;
   uint32_t nrcv = 0, nsamp = 0;
   uint64_t nprim = 0;
-  uint32_t stack_depth = MAX_STACK;
+  uint32_t stack_depth = MAX_STACK;  // traversal stack bound of the scene's 4-wide nodes
+  uint32_t lds_stack_cap = 0;        // 16, 24 or 32: LDS stack entries at most (0: MAX_STACK)
   // solve buffers
   DevBuf start_hist;  // per-instance histogram / offsets of the first realisations
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, scan_chunks, counters,
-      tallies, jobs;
+      tallies, jobs, trace_ovf;
   DevBuf adj_state, adj_in, adj_grad;
   DevBuf trk_vtx, trk_type, trk_seq, trk_count, trk_pos;
   uint64_t trk_paths = 0;           // realisations of the last tracked solve
@@ -1715,9 +1788,9 @@ namespace {
 int init_resident_grids(ssolh_context* c, uint32_t cus) {
   const void* fns[] = {
       (const void*)k_trace<16, true>, (const void*)k_trace<24, true>,
-      (const void*)k_trace<32, true>, (const void*)k_trace<MAX_STACK, true>,
+      (const void*)k_trace<MAX_STACK, true>,
       (const void*)k_trace32<16, true>, (const void*)k_trace32<24, true>,
-      (const void*)k_trace32<32, true>, (const void*)k_trace32<MAX_STACK, true>,
+      (const void*)k_trace32<MAX_STACK, true>,
       (const void*)k_shade<false, false, true>, (const void*)k_shade<false, true, true>,
       (const void*)k_shade<true, false, true>, (const void*)k_shade<true, true, true>,
       (const void*)k_shade32<false, true>, (const void*)k_shade32<true, true>,
@@ -1835,6 +1908,105 @@ int build_iss(const ssolh_scene* s, std::vector<IssInfo>& out) {
   return 0;
 }
 
+// LDS stack entries per lane of the trace kernel: the smallest of 16, 24, 32
+// that holds the scene's bound (the more blocks then share a CU), within the
+// context's cap; the rest of the bound goes to the overflow array
+uint32_t trace_lds_stack(const ssolh_context* c) {
+  uint32_t e = c->stack_depth <= 16 ? 16 : c->stack_depth <= 24 ? 24 : (uint32_t)MAX_STACK;
+  if (c->lds_stack_cap && c->lds_stack_cap < e) e = c->lds_stack_cap;
+  return e;
+}
+
+// BVH2 -> 4-wide nodes. A 4-wide node takes the children of a BVH2 node
+// and, while it holds fewer than four and one of them is an inner node,
+// replaces the inner child of largest box area by that child's children;
+// each inner child left becomes a 4-wide node of its own. pb[i]: the most
+// stack entries a descent from node i pushes (each visited node pushes all
+// but one of the children a ray crosses).
+struct Collapse4 {
+  struct Ent {
+    float lo[3], hi[3];
+    uint32_t child, count;  // count 0: inner BVH2 node `child`
+  };
+  const ssolh_bvh_node* in;
+  uint32_t nin;
+  std::vector<Node4> out;
+  std::vector<uint32_t> pb;
+  int err = 0;
+  char msg[160] = "";
+
+  Collapse4(const ssolh_bvh_node* nodes, uint32_t n) : in(nodes), nin(n) {}
+
+  static float area(const Ent& e) {
+    const float x = e.hi[0] - e.lo[0], y = e.hi[1] - e.lo[1], z = e.hi[2] - e.lo[2];
+    return x * y + y * z + z * x;
+  }
+  void fail(int code, const char* fmt, uint32_t a, uint32_t b) {
+    if (!err) snprintf(msg, sizeof(msg), fmt, a, b);
+    err = code;
+  }
+  // children of BVH2 node b appended to e[n..]
+  void take(uint32_t b, Ent* e, int& n) {
+    const ssolh_bvh_node& m = in[b];
+    for (int k = 0; k < 2; ++k) {
+      if (m.count[k] == SSOLH_BVH_EMPTY) continue;
+      Ent& x = e[n++];
+      for (int a = 0; a < 3; ++a) { x.lo[a] = m.lo[k][a]; x.hi[a] = m.hi[k][a]; }
+      x.child = m.child[k];
+      x.count = m.count[k];
+    }
+  }
+  uint32_t root(uint32_t b) { return node(b, 0); }
+  uint32_t node(uint32_t b, int depth) {
+    if (b >= nin) { fail(SSOLH_ERR_ARG, "BVH node %u out of %u nodes", b, nin); return 0; }
+    if (depth > 256) { fail(SSOLH_ERR_LIMIT, "BVH deeper than %u levels at node %u", 256, b); return 0; }
+    Ent e[5];
+    int n = 0;
+    take(b, e, n);
+    while (n < 4) {
+      int best = -1;
+      float ba = -1.f;
+      for (int i = 0; i < n; ++i)
+        if (e[i].count == 0 && area(e[i]) > ba) { ba = area(e[i]); best = i; }
+      if (best < 0) break;
+      const uint32_t b2 = e[best].child;
+      if (b2 >= nin) { fail(SSOLH_ERR_ARG, "BVH child %u out of %u nodes", b2, nin); return 0; }
+      e[best] = e[--n];
+      take(b2, e, n);
+    }
+    const uint32_t me = (uint32_t)out.size();
+    out.emplace_back();
+    pb.push_back(0);
+    uint32_t ref[4], deeper = 0;
+    for (int i = 0; i < 4; ++i) {
+      if (i >= n) { ref[i] = EMPTY4; continue; }
+      if (e[i].count == 0) {
+        ref[i] = node(e[i].child, depth + 1);
+        if (err) return 0;
+        deeper = std::max(deeper, pb[ref[i]]);
+      } else {
+        if (e[i].count > 8 || e[i].child >= (1u << 28)) {
+          fail(SSOLH_ERR_LIMIT, "BVH leaf of %u items at %u (at most 8 below 2^28)", e[i].count,
+               e[i].child);
+          return 0;
+        }
+        ref[i] = LEAF | ((e[i].count - 1u) << 28) | e[i].child;
+      }
+    }
+    Node4& o = out[me];
+    for (int i = 0; i < 4; ++i) {
+      for (int a = 0; a < 3; ++a) {
+        o.lo[a][i] = i < n ? e[i].lo[a] : 0.f;
+        o.hi[a][i] = i < n ? e[i].hi[a] : 0.f;
+      }
+      o.child[i] = ref[i];
+      o.pad[i] = 0;
+    }
+    pb[me] = (n > 1 ? (uint32_t)n - 1u : 0u) + deeper;
+    return me;
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -1906,7 +2078,7 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
   free_scene(c);
   for (DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
                     &c->block_emits, &c->block_offsets, &c->scan_chunks, &c->counters, &c->tallies,
-                    &c->jobs, &c->start_hist,
+                    &c->jobs, &c->start_hist, &c->trace_ovf,
                     &c->adj_state, &c->adj_in, &c->adj_grad, &c->trk_vtx, &c->trk_type,
                     &c->trk_seq, &c->trk_count, &c->trk_pos})
     if (b->p) (void)hipFree(b->p);
@@ -1922,11 +2094,6 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
     set_err("ssolh_scene_upload: empty scene (sampled triangles %u, instances %u)",
             s->nsample_tris, s->ninstances);
     return SSOLH_ERR_ARG;
-  }
-  if (s->stack_depth < 2 || s->stack_depth > (uint32_t)MAX_STACK) {
-    set_err("ssolh_scene_upload: traversal stack depth %u outside [2, %d]", s->stack_depth,
-            MAX_STACK);
-    return SSOLH_ERR_LIMIT;
   }
   std::vector<ShapeQ> shapeq(s->nshapes);
   for (uint32_t i = 0; i < s->nshapes; ++i) {
@@ -1957,58 +2124,53 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   r |= upload(c, s->media, s->nmedia, &d.media);
   r |= upload(c, s->materials, s->nmaterials, &d.mtls);
   r |= upload(c, s->shapes, s->nshapes, &d.shapes);
-  r |= upload(c, s->objects, s->nobjects, &d.objects);
-  // Device copy of the nodes with each child reference in the form the
-  // traversal follows: an inner child stays its node index, a leaf becomes
-  // LEAF | (count - 1) << 28 | first item (count stays; SSOLH_BVH_EMPTY
-  // still marks an empty child)
-  std::vector<ssolh_bvh_node> dnodes(s->nodes, s->nodes + s->nnodes);
-  if (s->nnodes >= TOP_REF) { set_err("ssolh_scene_upload: %u BVH nodes", s->nnodes); return SSOLH_ERR_LIMIT; }
-  for (ssolh_bvh_node& n : dnodes)
-    for (int k = 0; k < 2; ++k) {
-      if (n.count[k] == SSOLH_BVH_EMPTY) continue;
-      if (n.count[k] == 0) {
-        if (n.child[k] >= s->nnodes) {
-          set_err("ssolh_scene_upload: BVH child %u out of %u nodes", n.child[k], s->nnodes);
-          return SSOLH_ERR_ARG;
-        }
-        continue;
-      }
-      if (n.count[k] > 8 || n.child[k] >= (1u << 28)) {
-        set_err("ssolh_scene_upload: BVH leaf of %u items at %u (at most 8 below 2^28)",
-                n.count[k], n.child[k]);
-        return SSOLH_ERR_LIMIT;
-      }
-      n.child[k] = LEAF | ((n.count[k] - 1u) << 28) | n.child[k];
-    }
-  r |= upload(c, dnodes.data(), dnodes.size(), &d.nodes);
+  // 4-wide traversal nodes from the host's BVH2: each object's BLAS (one
+  // copy per distinct root) and the TLAS, with the objects' roots remapped
+  Collapse4 col(s->nodes, s->nnodes);
+  std::vector<ssolh_object> objects(s->objects, s->objects + s->nobjects);
+  std::map<uint32_t, uint32_t> blas_of;  // BVH2 root -> 4-wide root
+  uint32_t blas_pb = 0;
+  for (ssolh_object& o : objects) {
+    auto it = blas_of.find(o.blas_root);
+    if (it == blas_of.end()) it = blas_of.emplace(o.blas_root, col.root(o.blas_root)).first;
+    o.blas_root = it->second;
+    blas_pb = std::max(blas_pb, col.pb[it->second]);
+  }
+  const uint32_t troot = col.root(s->tlas_root);
+  if (col.err) { set_err("ssolh_scene_upload: %s", col.msg); return col.err; }
+  if (col.out.size() >= TOP_REF) {
+    set_err("ssolh_scene_upload: %zu BVH nodes", col.out.size());
+    return SSOLH_ERR_LIMIT;
+  }
+  r |= upload(c, objects.data(), objects.size(), &d.objects);
+  r |= upload(c, col.out.data(), col.out.size(), &d.nodes);
   {
-    // the TLAS's first TOP_NODES inner nodes, breadth first, with the
-    // references between them rewritten to TOP_REF | index
-    std::vector<ssolh_bvh_node> top;
-    std::vector<uint32_t> order;
-    std::vector<int32_t> where;  // global node -> top index
-    if (s->tlas_root < s->nnodes) {
-      order.push_back(s->tlas_root);
-      for (size_t h = 0; h < order.size() && order.size() < (size_t)TOP_NODES; ++h) {
-        const ssolh_bvh_node& n = dnodes[order[h]];
-        for (int k = 0; k < 2 && order.size() < (size_t)TOP_NODES; ++k)
-          if (n.count[k] == 0) order.push_back(n.child[k]);
+    // the TLAS's first TOP_NODES nodes, breadth first, with the references
+    // between them rewritten to TOP_REF | index
+    std::vector<uint32_t> order{troot};
+    for (size_t h = 0; h < order.size() && order.size() < (size_t)TOP_NODES; ++h)
+      for (int k = 0; k < 4 && order.size() < (size_t)TOP_NODES; ++k) {
+        const uint32_t ch = col.out[order[h]].child[k];
+        if (!(ch & LEAF)) order.push_back(ch);
       }
-    }
-    where.assign(s->nnodes, -1);
-    for (size_t i = 0; i < order.size(); ++i) where[order[i]] = (int32_t)i;
+    std::map<uint32_t, uint32_t> where;
+    for (size_t i = 0; i < order.size(); ++i) where[order[i]] = (uint32_t)i;
+    std::vector<Node4> top;
     for (uint32_t g : order) {
-      ssolh_bvh_node n = dnodes[g];
-      for (int k = 0; k < 2; ++k)
-        if (n.count[k] == 0 && where[n.child[k]] >= 0)
-          n.child[k] = TOP_REF | (uint32_t)where[n.child[k]];
+      Node4 n = col.out[g];
+      for (int k = 0; k < 4; ++k) {
+        auto w = (n.child[k] & LEAF) ? where.end() : where.find(n.child[k]);
+        if (w != where.end()) n.child[k] = TOP_REF | w->second;
+      }
       top.push_back(n);
     }
     r |= upload(c, top.data(), top.size(), &d.top);
     d.ntop = (uint32_t)top.size();
-    d.root_ref = top.empty() ? s->tlas_root : TOP_REF;
+    d.root_ref = TOP_REF;
   }
+  // a lane's stack: the TLAS descent's pushes, the instance sentinel, the
+  // deepest BLAS descent's pushes
+  c->stack_depth = col.pb[troot] + 1 + blas_pb;
   r |= upload(c, s->triangles, s->ntriangles, &d.tris);
   r |= upload(c, s->instances, s->ninstances, &d.insts);
   r |= upload(c, s->tlas_instances, s->ninstances, &d.tlas_inst);
@@ -2019,7 +2181,7 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   r |= upload(c, s->sample_alias, s->nsample_tris, &d.alias);
   if (r) return r;
   d.air = s->air_medium;
-  d.tlas_root = s->tlas_root;
+  d.tlas_root = troot;
   d.nstris = s->nsample_tris;
   d.sun = s->sun;
   d.S = s->sampled_area;
@@ -2029,7 +2191,6 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   c->nrcv = s->nreceivers;
   c->nsamp = s->nsampled;
   c->nprim = s->nprim_tallies;
-  c->stack_depth = s->stack_depth;
   c->ninstances = s->ninstances;
   HIP_TRY(hipStreamSynchronize(c->stream));  // the host arrays may go after return
   c->has_scene = true;
@@ -2212,6 +2373,15 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   if ((r = alloc(c->scan_chunks, ((max_blocks + SCAN_CHUNK - 1) / SCAN_CHUNK) * sizeof(uint32_t))))
     return r;
   if ((r = alloc(c->counters, sizeof(Counters) + 4 * sizeof(unsigned)))) return r;
+  // traversal stack entries past the LDS ones (scenes whose bound exceeds
+  // MAX_STACK), per work-list position
+  const uint32_t lds_stack = trace_lds_stack(c);
+  TOvf ovf{nullptr, slots, 0};
+  if (c->stack_depth > lds_stack) {
+    ovf.extra = c->stack_depth - lds_stack;
+    if ((r = alloc(c->trace_ovf, n * ovf.extra * sizeof(uint32_t)))) return r;
+    ovf.base = (uint32_t*)c->trace_ovf.p;
+  }
   // adjoint buffers
   AdjParams A{};
   if (adj) {
@@ -2348,14 +2518,12 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       HitRec32* h32 = (HitRec32*)c->hits.p;
       auto launch = [&](auto kernel) {
         kernel<<<grid_of((const void*)kernel), BLOCK, 0, c->stream>>>(c->scn, c->shapeq, list,
-                                                                     count, r32, h32, fetch);
+                                                                     count, r32, h32, fetch, ovf);
       };
-      if (c->stack_depth <= 16)
+      if (lds_stack == 16)
         loop ? launch(k_trace32<16, true>) : launch(k_trace32<16, false>);
-      else if (c->stack_depth <= 24)
+      else if (lds_stack == 24)
         loop ? launch(k_trace32<24, true>) : launch(k_trace32<24, false>);
-      else if (c->stack_depth <= 32)
-        loop ? launch(k_trace32<32, true>) : launch(k_trace32<32, false>);
       else
         loop ? launch(k_trace32<MAX_STACK, true>) : launch(k_trace32<MAX_STACK, false>);
     } else {
@@ -2363,14 +2531,12 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       HitRec* h64 = (HitRec*)c->hits.p;
       auto launch = [&](auto kernel) {
         kernel<<<grid_of((const void*)kernel), BLOCK, 0, c->stream>>>(c->scn, list, count, r64,
-                                                                     h64, fetch);
+                                                                     h64, fetch, ovf);
       };
-      if (c->stack_depth <= 16)
+      if (lds_stack == 16)
         loop ? launch(k_trace<16, true>) : launch(k_trace<16, false>);
-      else if (c->stack_depth <= 24)
+      else if (lds_stack == 24)
         loop ? launch(k_trace<24, true>) : launch(k_trace<24, false>);
-      else if (c->stack_depth <= 32)
-        loop ? launch(k_trace<32, true>) : launch(k_trace<32, false>);
       else
         loop ? launch(k_trace<MAX_STACK, true>) : launch(k_trace<MAX_STACK, false>);
     }
@@ -2643,7 +2809,8 @@ SSOLH_API int ssolh_solve_noalloc(ssolh_context* c, const struct ssolh_solve_arg
   // put back after it.
   DevBuf* bufs[] = {&c->state,        &c->rays,         &c->hits,        &c->lists[0],
                     &c->lists[1],     &c->emitted,      &c->block_emits, &c->block_offsets,
-                    &c->scan_chunks,  &c->counters,     &c->tallies,     &c->start_hist};
+                    &c->scan_chunks,  &c->counters,     &c->tallies,     &c->start_hist,
+                    &c->trace_ovf};
   constexpr size_t NB = sizeof(bufs) / sizeof(bufs[0]);
   DevBuf saved[NB];
   for (size_t i = 0; i < NB; ++i) {
@@ -2894,6 +3061,16 @@ SSOLH_API int ssolh_test_wave_sum(ssolh_context* c, uint32_t nwaves, const doubl
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(out, d_out, bytes, hipMemcpyDeviceToHost));
   (void)hipFree(d_in); (void)hipFree(d_out);
+  return 0;
+}
+
+SSOLH_API int ssolh_test_trace_stack(ssolh_context* c, uint32_t lds_cap, uint32_t* bound) {
+  if (!c || !(lds_cap == 0 || lds_cap == 16 || lds_cap == 24 || lds_cap == 32)) {
+    set_err("ssolh_test_trace_stack: cap %u not 0, 16, 24 or 32", lds_cap);
+    return SSOLH_ERR_ARG;
+  }
+  c->lds_stack_cap = lds_cap;
+  if (bound) *bound = c->has_scene ? c->stack_depth : 0;
   return 0;
 }
 

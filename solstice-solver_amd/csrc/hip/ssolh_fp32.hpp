@@ -170,7 +170,7 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
                                             const uint32_t* __restrict__ list, uint32_t q,
                                             const RayRec32* __restrict__ rays,
                                             HitRec32* __restrict__ hits, uint32_t* lds_stack,
-                                            const ssolh_bvh_node* lds_top) {
+                                            const Node4* lds_top, const TOvf& ovf) {
 
   const uint32_t slot = slot_of(list, q);
   const RayRec32 r = rays[slot];
@@ -186,8 +186,7 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
   ts.qdst = INFINITY;
   ts.qiss = NONE;
 
-  uint32_t* stack = lds_stack + threadIdx.x;
-  int sp = 0;
+  TStack st{lds_stack + threadIdx.x, ovf.base ? ovf.base + q : nullptr, ovf.stride, ovf.extra, 0};
   f3 o = ow, d = dw;           // current-level ray
   FRay fr = FRay{ow.x, ow.y, ow.z, 0, 0, 0};
   {
@@ -208,35 +207,11 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
 
   uint32_t ref = S.root_ref;  // the TLAS's top levels come from the LDS copy (as trace_ray)
   bool alive = true;
-  auto pop = [&]() -> bool {
-    if (sp == 0) return false;
-    --sp;
-    ref = stack[sp * BLOCK];
-    return true;
-  };
   float tmax = FLT_MAX;  // box culling distance (as trace_ray)
   while (alive) {
     while (!(ref & LEAF)) {
-      ssolh_bvh_node n;  // one round of loads, then both boxes
-      if (ref & TOP_REF)
-        n = lds_node(lds_top, ref & 0x0FFFFFFFu);
-      else
-        n = S.nodes[ref];
-      float tn0, tn1;
-      const bool b0 = box_hit(fr, n.lo[0], n.hi[0], ftmin, tmax, tn0);
-      const bool b1 = box_hit(fr, n.lo[1], n.hi[1], ftmin, tmax, tn1);
-      const bool h0 = b0 & (n.count[0] != SSOLH_BVH_EMPTY);
-      const bool h1 = b1 & (n.count[1] != SSOLH_BVH_EMPTY);
-      if (h0 && h1) {
-        const bool near0 = tn0 <= tn1;
-        const uint32_t rn = near0 ? n.child[0] : n.child[1], rf = near0 ? n.child[1] : n.child[0];
-        if (sp < STACK) { stack[sp * BLOCK] = rf; ++sp; }
-        ref = rn;
-      } else if (h0) {
-        ref = n.child[0];
-      } else if (h1) {
-        ref = n.child[1];
-      } else if (!pop()) {
+      const Node4 n = (ref & TOP_REF) ? lds_node4(lds_top, ref & 0x0FFFFFFFu) : S.nodes[ref];
+      if (!visit4<STACK>(n, fr, ftmin, tmax, ref, st)) {
         alive = false;
         break;
       }
@@ -245,14 +220,14 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
     if (ref == SENTINEL) {
       in_blas = false;
       fr = fw;
-      alive = pop();
+      alive = tpop<STACK>(st, ref);
       continue;
     }
     const uint32_t first = ref & 0x0FFFFFFFu;
     if (!in_blas) {
       const uint32_t ii = S.tlas_inst[first];
       if (ii == skip_inst) {
-        alive = pop();
+        alive = tpop<STACK>(st, ref);
         continue;
       }
       d3 od, dd;
@@ -263,7 +238,7 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
       sh = make_shear32(d);
       iss_first = S.insts[ii].isshape_first;
       in_blas = true;
-      if (sp < STACK) { stack[sp * BLOCK] = SENTINEL; ++sp; }
+      tpush<STACK>(st, SENTINEL);
       ref = S.objects[S.insts[ii].object].blas_root;
       continue;
     }
@@ -286,7 +261,7 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
       if (any_hit) { done = true; break; }
     }
     if (done) break;
-    alive = pop();
+    alive = tpop<STACK>(st, ref);
   }
 
   HitRec32 h;
@@ -327,9 +302,9 @@ template <int STACK, bool LOOP>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_trace32(DScene S, const ShapeQ* __restrict__ SQ, const uint32_t* __restrict__ list,
           const unsigned* __restrict__ count_ptr, const RayRec32* __restrict__ rays,
-          HitRec32* __restrict__ hits, unsigned* fetch) {
+          HitRec32* __restrict__ hits, unsigned* fetch, TOvf ovf) {
   __shared__ uint32_t lds_stack[STACK * BLOCK];
-  __shared__ ssolh_bvh_node lds_top[TOP_NODES];
+  __shared__ Node4 lds_top[TOP_NODES];
   const uint32_t count = *count_ptr;
   if (blockIdx.x * BLOCK >= count) return;  // whole blocks only: the staging syncs
   for (uint32_t i = threadIdx.x; i < S.ntop; i += BLOCK) lds_top[i] = S.top[i];
@@ -337,11 +312,11 @@ k_trace32(DScene S, const ShapeQ* __restrict__ SQ, const uint32_t* __restrict__ 
   if (LOOP) {  // graph-captured launch of a fixed grid (as k_trace)
     for_each_block((count + BLOCK - 1) / BLOCK, fetch, [&](uint32_t b) {
       const uint32_t q = b * BLOCK + threadIdx.x;
-      if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack, lds_top);
+      if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack, lds_top, ovf);
     });
   } else {
     const uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-    if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack, lds_top);
+    if (q < count) trace_ray32<STACK>(S, SQ, list, q, rays, hits, lds_stack, lds_top, ovf);
   }
 }
 

@@ -22,7 +22,9 @@ using namespace ssol;
 
 namespace {
 
-constexpr int GPU_STACK = 48;  // traversal stack entries of the trace kernel
+// deepest TLAS + BLAS descent accepted: the GPU traversal keeps its first
+// stack entries in LDS and the rest, for deep trees, in device memory
+constexpr int MAX_BVH_LEVELS = 200;
 
 struct DataKey {
   int spectrum;
@@ -622,8 +624,8 @@ res_T build_tlas(ssol_device* dev, Flat& f) {
   }
   unsigned tdepth = 0;
   f.scn.tlas_root = bvh_build(boxes, 1, f.nodes, f.tlas_inst, &tdepth);
-  if (tdepth + max_blas + 2 > (unsigned)GPU_STACK) {
-    log_error(dev, "ssol_solve: BVH too deep for the GPU traversal stack (%u + %u levels).\n",
+  if (tdepth + max_blas + 2 > (unsigned)MAX_BVH_LEVELS) {
+    log_error(dev, "ssol_solve: BVH too deep (%u + %u levels).\n",
               tdepth, max_blas);
     return RES_MEM_ERR;
   }

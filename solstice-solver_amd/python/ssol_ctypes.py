@@ -22,7 +22,8 @@ import numpy as np
 import ssol_scene as S
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_DIR = os.path.join(os.path.dirname(_HERE), "lib")
+# the built libraries; SSOL_LIB_DIR points at another build of them
+LIB_DIR = os.environ.get("SSOL_LIB_DIR") or os.path.join(os.path.dirname(_HERE), "lib")
 _LIB = None
 
 RES = {0: "RES_OK", 1: "RES_BAD_ARG", 2: "RES_MEM_ERR", 3: "RES_IO_ERR",
