@@ -95,15 +95,17 @@ struct alignas(8) IssInfo {
   int64_t prim_tally;      // first per-primitive slot or -1
 };
 
-// BVH node of the traversal: four child boxes (by axis, then child) and
-// four child references in the form the traversal follows (inner node
-// index, TOP_REF | index into the LDS copy, LEAF | (count - 1) << 28 | first
-// item, or EMPTY4). Built at upload by collapsing the host's BVH2 (each
-// node takes the children and grandchildren of a BVH2 node, opening the
-// largest child boxes first), which halves the depth a ray descends.
+// BVH node of the traversal: four child boxes as one 16-byte row per axis
+// and side (box[axis][0] the four lower planes, box[axis][1] the upper
+// ones; an absent child has lower planes +inf and upper planes -inf, which
+// no ray crosses) and four child references in the form the traversal
+// follows (inner node index, TOP_REF | index into the LDS copy,
+// LEAF | (count - 1) << 28 | first item, or EMPTY4). Built at upload by
+// collapsing the host's BVH2 (each node takes the children and grandchildren
+// of a BVH2 node, opening the largest child boxes first), which halves the
+// depth a ray descends.
 struct alignas(16) Node4 {
-  float lo[3][4];
-  float hi[3][4];
+  float box[3][2][4];
   uint32_t child[4];
   uint32_t pad[4];
 };
@@ -417,27 +419,42 @@ __device__ void block_count(bool pred, uint32_t* out, uint32_t b) {
 // -------------------------------------------------------------------------
 // Traversal with the reference hit filter
 // -------------------------------------------------------------------------
+// Ray of the box tests: the distance to the plane x = p is p * ix + ox (one
+// FMA), and the rows of a node that hold the near planes of the ray's
+// direction signs are read by offset, so a slab needs no min/max. ox, oy, oz
+// are kept finite; a ray with a non-finite origin or direction gets
+// ox = oy = oz = +inf and crosses no box.
 struct FRay {
-  float ox, oy, oz, ix, iy, iz;
+  float ix, iy, iz;      // 1 / direction (components at least 1e-30 in magnitude)
+  float ox, oy, oz;      // -origin / direction
+  uint32_t nx, ny, nz;   // byte offset in a Node4 of each axis's near-plane row
 };
 
-__device__ __forceinline__ FRay make_fray(d3 o, d3 d) {
-  auto inv = [](double x) -> float {
-    const float f = (float)x;
-    return 1.0f / (fabsf(f) > 1e-30f ? f : copysignf(1e-30f, f));
-  };
-  return FRay{(float)o.x, (float)o.y, (float)o.z, inv(d.x), inv(d.y), inv(d.z)};
+__device__ __forceinline__ FRay make_fray32(float ox, float oy, float oz, float dx, float dy,
+                                            float dz) {
+  auto inv = [](float f) { return 1.0f / (fabsf(f) > 1e-30f ? f : copysignf(1e-30f, f)); };
+  auto off = [](float o, float i) { return fminf(fmaxf(-o * i, -FLT_MAX), FLT_MAX); };
+  FRay r;
+  r.ix = inv(dx);
+  r.iy = inv(dy);
+  r.iz = inv(dz);
+  r.ox = off(ox, r.ix);
+  r.oy = off(oy, r.iy);
+  r.oz = off(oz, r.iz);
+  r.nx = r.ix < 0.f ? 16u : 0u;
+  r.ny = r.iy < 0.f ? 48u : 32u;
+  r.nz = r.iz < 0.f ? 80u : 64u;
+  const bool finite = fabsf(ox) <= FLT_MAX && fabsf(oy) <= FLT_MAX && fabsf(oz) <= FLT_MAX &&
+                      fabsf(dx) <= FLT_MAX && fabsf(dy) <= FLT_MAX && fabsf(dz) <= FLT_MAX;
+  if (!finite) {
+    r.ix = r.iy = r.iz = 1.f;
+    r.ox = r.oy = r.oz = INFINITY;
+  }
+  return r;
 }
 
-__device__ __forceinline__ bool box_hit(const FRay& r, const float* lo, const float* hi,
-                                        float tmin, float tmax, float& tnear) {
-  const float t0x = (lo[0] - r.ox) * r.ix, t1x = (hi[0] - r.ox) * r.ix;
-  const float t0y = (lo[1] - r.oy) * r.iy, t1y = (hi[1] - r.oy) * r.iy;
-  const float t0z = (lo[2] - r.oz) * r.iz, t1z = (hi[2] - r.oz) * r.iz;
-  const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-  const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
-  tnear = tn;
-  return tn <= tf;
+__device__ __forceinline__ FRay make_fray(d3 o, d3 d) {
+  return make_fray32((float)o.x, (float)o.y, (float)o.z, (float)d.x, (float)d.y, (float)d.z);
 }
 
 // Ray in the shear frame of the watertight ray/triangle test (Woop,
@@ -541,25 +558,49 @@ __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t is
   return true;
 }
 
-// Node i of the LDS copy of the top levels, read through an LDS (address
-// space 3) pointer: the load is a ds_read, where a pointer that may address
-// either LDS or global memory would make it a flat access
-__device__ __forceinline__ Node4 lds_node4(const Node4* lds, uint32_t i) {
-  typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-  typedef const __attribute__((address_space(3))) v4u* lds_u4;
-  const lds_u4 p = (lds_u4)(lds + i);
-  Node4 n;
-  v4u* w = (v4u*)&n;
-  for (int k = 0; k < (int)(sizeof(n) / 16); ++k) w[k] = p[k];
-  return n;
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
+// What a visit reads of a node: the near- and far-plane rows of the ray's
+// direction signs and the child references
+struct NodeRows {
+  v4f n[3], f[3];
+  v4u child;
+};
+
+// Rows of node `ref` of the device array (32-bit byte offsets from the
+// array's base: the loads take the base in scalar registers)
+__device__ __forceinline__ NodeRows node_rows(const Node4* __restrict__ nodes, uint32_t ref,
+                                              const FRay& r) {
+  const char* b = (const char*)nodes;
+  const uint32_t base = ref * (uint32_t)sizeof(Node4);
+  const uint32_t o[3] = {base + r.nx, base + r.ny, base + r.nz};
+  NodeRows w;
+  for (int a = 0; a < 3; ++a) {
+    w.n[a] = *(const v4f*)(b + o[a]);
+    w.f[a] = *(const v4f*)(b + (o[a] ^ 16u));
+  }
+  w.child = *(const v4u*)(b + base + 96u);
+  return w;
 }
 
-// Child c's box of a 4-wide node (the arithmetic of box_hit)
-__device__ __forceinline__ bool box_hit4(const FRay& r, const Node4& n, int c, float tmin,
-                                         float tmax, float& tnear) {
-  const float lo[3] = {n.lo[0][c], n.lo[1][c], n.lo[2][c]};
-  const float hi[3] = {n.hi[0][c], n.hi[1][c], n.hi[2][c]};
-  return box_hit(r, lo, hi, tmin, tmax, tnear);
+// The same from the LDS copy of the top levels, through an LDS (address
+// space 3) pointer: the loads are ds_reads, where a pointer that may address
+// either LDS or global memory would make them flat accesses
+__device__ __forceinline__ NodeRows lds_rows(const Node4* lds, uint32_t i, const FRay& r) {
+  typedef const __attribute__((address_space(3))) char* lds_b;
+  const lds_b b = (lds_b)lds;
+  const uint32_t base = i * (uint32_t)sizeof(Node4);
+  const uint32_t o[3] = {base + r.nx, base + r.ny, base + r.nz};
+  typedef const __attribute__((address_space(3))) v4f* lf;
+  typedef const __attribute__((address_space(3))) v4u* lu;
+  NodeRows w;
+  for (int a = 0; a < 3; ++a) {
+    w.n[a] = *(lf)(b + o[a]);
+    w.f[a] = *(lf)(b + (o[a] ^ 16u));
+  }
+  w.child = *(lu)(b + base + 96u);
+  return w;
 }
 
 // Traversal stack of one lane: its first STACK entries in LDS (stride
@@ -598,15 +639,17 @@ __device__ __forceinline__ bool tpop(TStack& s, uint32_t& ref) {
 // others are pushed farthest first (so the next nearest is on top). With no
 // child crossed, `ref` is popped; false when the stack is empty.
 template <int STACK>
-__device__ __forceinline__ bool visit4(const Node4& n, const FRay& fr, float tmin, float tmax,
+__device__ __forceinline__ bool visit4(const NodeRows& w, const FRay& fr, float tmin, float tmax,
                                        uint32_t& ref, TStack& st) {
   float k[4];
   uint32_t r[4];
   for (int c = 0; c < 4; ++c) {
-    float tn;
-    const bool h = box_hit4(fr, n, c, tmin, tmax, tn) & (n.child[c] != EMPTY4);
-    k[c] = h ? tn : INFINITY;
-    r[c] = n.child[c];
+    const float tn = fmaxf(fmaxf(fmaf(w.n[0][c], fr.ix, fr.ox), fmaf(w.n[1][c], fr.iy, fr.oy)),
+                           fmaxf(fmaf(w.n[2][c], fr.iz, fr.oz), tmin));
+    const float tf = fminf(fminf(fmaf(w.f[0][c], fr.ix, fr.ox), fmaf(w.f[1][c], fr.iy, fr.oy)),
+                           fminf(fmaf(w.f[2][c], fr.iz, fr.oz), tmax));
+    k[c] = tn <= tf ? tn : INFINITY;
+    r[c] = w.child[c];
   }
   // sorting network (0,1)(2,3)(0,2)(1,3)(1,2)
   auto cs = [&](int a, int b) {
@@ -685,7 +728,8 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
     while (!(ref & LEAF)) {
       // the whole node in one round of loads (LDS or global memory, each
       // through its own address space), then its four boxes
-      const Node4 n = (ref & TOP_REF) ? lds_node4(lds_top, ref & 0x0FFFFFFFu) : S.nodes[ref];
+      const NodeRows n =
+          (ref & TOP_REF) ? lds_rows(lds_top, ref & 0x0FFFFFFFu, fr) : node_rows(S.nodes, ref, fr);
       if (!visit4<STACK>(n, fr, ftmin, tmax, ref, st)) {
         alive = false;
         break;
@@ -1996,8 +2040,8 @@ struct Collapse4 {
     Node4& o = out[me];
     for (int i = 0; i < 4; ++i) {
       for (int a = 0; a < 3; ++a) {
-        o.lo[a][i] = i < n ? e[i].lo[a] : 0.f;
-        o.hi[a][i] = i < n ? e[i].hi[a] : 0.f;
+        o.box[a][0][i] = i < n ? e[i].lo[a] : INFINITY;
+        o.box[a][1][i] = i < n ? e[i].hi[a] : -INFINITY;
       }
       o.child[i] = ref[i];
       o.pad[i] = 0;
@@ -2138,7 +2182,7 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   }
   const uint32_t troot = col.root(s->tlas_root);
   if (col.err) { set_err("ssolh_scene_upload: %s", col.msg); return col.err; }
-  if (col.out.size() >= TOP_REF) {
+  if (col.out.size() >= (1u << 25)) {  // byte offsets of the nodes fit 32 bits
     set_err("ssolh_scene_upload: %zu BVH nodes", col.out.size());
     return SSOLH_ERR_LIMIT;
   }

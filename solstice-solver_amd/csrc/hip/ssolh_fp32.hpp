@@ -188,11 +188,7 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
 
   TStack st{lds_stack + threadIdx.x, ovf.base ? ovf.base + q : nullptr, ovf.stride, ovf.extra, 0};
   f3 o = ow, d = dw;           // current-level ray
-  FRay fr = FRay{ow.x, ow.y, ow.z, 0, 0, 0};
-  {
-    auto inv = [](float x) { return 1.0f / (fabsf(x) > 1e-30f ? x : copysignf(1e-30f, x)); };
-    fr.ix = inv(dw.x); fr.iy = inv(dw.y); fr.iz = inv(dw.z);
-  }
+  FRay fr = make_fray32(ow.x, ow.y, ow.z, dw.x, dw.y, dw.z);
   const FRay fw = fr;
   FShear sh = make_shear32(dw);
   // a ray leaving a planar object does not enter that object again
@@ -210,7 +206,8 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
   float tmax = FLT_MAX;  // box culling distance (as trace_ray)
   while (alive) {
     while (!(ref & LEAF)) {
-      const Node4 n = (ref & TOP_REF) ? lds_node4(lds_top, ref & 0x0FFFFFFFu) : S.nodes[ref];
+      const NodeRows n =
+          (ref & TOP_REF) ? lds_rows(lds_top, ref & 0x0FFFFFFFu, fr) : node_rows(S.nodes, ref, fr);
       if (!visit4<STACK>(n, fr, ftmin, tmax, ref, st)) {
         alive = false;
         break;
