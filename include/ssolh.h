@@ -226,6 +226,9 @@ struct ssolh_scene {
 
   uint32_t nobjects;
   const struct ssolh_object* objects;
+  const float* object_boxes;  /* [nobjects][6]: object-space lo xyz, hi xyz
+                                 (lo > hi: empty), the boxes the TLAS's
+                                 instance boxes are made of */
 
   uint32_t nnodes;
   const struct ssolh_bvh_node* nodes;   /* BLASes then TLAS */
@@ -330,6 +333,20 @@ SSOLH_API int ssolh_device_count(void);
 
 /* Copy the flat scene to the device (replaces the previous one) */
 SSOLH_API int ssolh_scene_upload(ssolh_context* ctx, const struct ssolh_scene* scn);
+
+/* Instances moved, nothing else changed (reference: the views rebuilt at
+ * every solve, src/ssol_scene.c:346-433): new instance records and
+ * (instance, shape) frames, same counts and objects as uploaded, replace the
+ * device copies, and the TLAS is refit on the GPU to the instances' new
+ * world boxes (same tree, new boxes; the LDS-staged top levels follow).
+ * *cost_ratio (if not NULL) = the refit tree's summed child-box surface area
+ * over the tree's as uploaded: the caller rebuilds and uploads the scene
+ * when the motion has made the tree too loose. Ordered on the context's
+ * stream; returns after the refit. */
+SSOLH_API int ssolh_scene_move_instances
+  (ssolh_context* ctx, uint32_t ninstances, const struct ssolh_instance* instances,
+   uint32_t ninst_sshapes, const struct ssolh_inst_sshape* inst_sshapes,
+   double* cost_ratio);
 
 /* Run the realisations of one solve over the uploaded scene */
 SSOLH_API int ssolh_solve(ssolh_context* ctx, const struct ssolh_solve_args* args);
@@ -456,6 +473,9 @@ SSOLH_API int ssolh_test_wave_sum
  * goes to the device-memory overflow; *bound (if not NULL) = the uploaded
  * scene's bound (0 without a scene). The results do not depend on the cap. */
 SSOLH_API int ssolh_test_trace_stack(ssolh_context* ctx, uint32_t lds_cap, uint32_t* bound);
+
+/* Scene uploads and TLAS refits this context has done */
+SSOLH_API int ssolh_test_scene_counters(ssolh_context* ctx, uint32_t* uploads, uint32_t* refits);
 
 /* Bandwidth of a plain device copy of `bytes` bytes repeated `reps` times
  * (read + write bytes over the HIP-event time), in GB/s. Calibrates the HBM

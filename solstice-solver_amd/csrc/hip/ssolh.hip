@@ -826,6 +826,97 @@ k_trace(DScene S, const uint32_t* __restrict__ list, const unsigned* __restrict_
 }
 
 // -------------------------------------------------------------------------
+// TLAS refit for moved instances (the reference rebuilds its scene views on
+// every solve, src/ssol_scene.c:346-433): the 4-wide TLAS keeps its
+// topology and takes the world boxes of the instances' new frames, level by
+// level from the deepest, in one workgroup; its LDS-staged copy of the top
+// levels follows. *cost = sum over the TLAS's child boxes of their surface
+// areas (x*y + y*z + z*x), the quality measure the host compares with the
+// tree's at build time.
+// -------------------------------------------------------------------------
+constexpr int REFIT_THREADS = 1024;
+
+// World box of instance ii: the 8 corners of its object's box under the
+// instance transform, in FP64 and in the order of the host's TLAS build,
+// rounded to FP32, widened by 2e-6 of its largest coordinate + 1e-6 (an
+// empty object gets the degenerate box at the origin, widened alike)
+__device__ void instance_world_box(const ssolh_instance& in, const float* ob, float lo[3],
+                                   float hi[3]) {
+  for (int k = 0; k < 3; ++k) { lo[k] = FLT_MAX; hi[k] = -FLT_MAX; }
+  if (ob[0] > ob[3]) {
+    for (int k = 0; k < 3; ++k) lo[k] = hi[k] = 0.f;
+  } else {
+    for (int c = 0; c < 8; ++c) {
+      const double p[3] = {(double)((c & 1) ? ob[3] : ob[0]), (double)((c & 2) ? ob[4] : ob[1]),
+                           (double)((c & 4) ? ob[5] : ob[2])};
+      for (int k = 0; k < 3; ++k) {
+        const double w = __dadd_rn(__dadd_rn(__dadd_rn(__dmul_rn(in.transform[k], p[0]),
+                                                       __dmul_rn(in.transform[3 + k], p[1])),
+                                             __dmul_rn(in.transform[6 + k], p[2])),
+                                   in.transform[9 + k]);
+        lo[k] = fminf(lo[k], (float)w);
+        hi[k] = fmaxf(hi[k], (float)w);
+      }
+    }
+  }
+  float m = 0.f;
+  for (int k = 0; k < 3; ++k) m = fmaxf(m, fmaxf(fabsf(lo[k]), fabsf(hi[k])));
+  const float e = __fadd_rn(__fmul_rn(m, 2e-6f), 1e-6f);
+  for (int k = 0; k < 3; ++k) { lo[k] = __fsub_rn(lo[k], e); hi[k] = __fadd_rn(hi[k], e); }
+}
+
+__global__ void __launch_bounds__(REFIT_THREADS)
+k_tlas_refit(const ssolh_instance* __restrict__ insts, const uint32_t* __restrict__ tlas_inst,
+             const float* __restrict__ obj_box, Node4* nodes, const uint32_t* __restrict__ order,
+             const uint32_t* __restrict__ level_end, uint32_t nlevels, Node4* top,
+             const uint32_t* __restrict__ top_src, uint32_t ntop, double* cost) {
+  __shared__ double part[REFIT_THREADS];
+  double acc = 0;
+  uint32_t begin = 0;
+  for (uint32_t l = 0; l < nlevels; ++l) {
+    const uint32_t end = level_end[l];
+    for (uint32_t i = begin + threadIdx.x; i < end; i += REFIT_THREADS) {
+      Node4& n = nodes[order[i]];
+      for (int c = 0; c < 4; ++c) {
+        const uint32_t ref = n.child[c];
+        if (ref == EMPTY4) continue;
+        float lo[3], hi[3];
+        if (ref & LEAF) {
+          const ssolh_instance& in = insts[tlas_inst[ref & 0x0FFFFFFFu]];
+          instance_world_box(in, obj_box + 6 * (size_t)in.object, lo, hi);
+        } else {
+          const Node4& m = nodes[ref];
+          for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(fminf(m.box[a][0][0], m.box[a][0][1]), fminf(m.box[a][0][2], m.box[a][0][3]));
+            hi[a] = fmaxf(fmaxf(m.box[a][1][0], m.box[a][1][1]), fmaxf(m.box[a][1][2], m.box[a][1][3]));
+          }
+        }
+        for (int a = 0; a < 3; ++a) {
+          n.box[a][0][c] = lo[a];
+          n.box[a][1][c] = hi[a];
+        }
+        const double x = (double)hi[0] - lo[0], y = (double)hi[1] - lo[1],
+                     z = (double)hi[2] - lo[2];
+        acc += x * y + y * z + z * x;
+      }
+    }
+    begin = end;
+    __syncthreads();  // the next level reads this one's boxes
+  }
+  for (uint32_t i = threadIdx.x; i < ntop; i += REFIT_THREADS)
+    for (int a = 0; a < 3; ++a)
+      for (int sd = 0; sd < 2; ++sd)
+        for (int c = 0; c < 4; ++c) top[i].box[a][sd][c] = nodes[top_src[i]].box[a][sd][c];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t w = REFIT_THREADS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *cost = part[0];
+}
+
+// -------------------------------------------------------------------------
 // Path generation (reference point_init, src/ssol_solver.c:207-358)
 // -------------------------------------------------------------------------
 struct Emit {
@@ -1795,6 +1886,18 @@ struct ssolh_context {
   uint64_t nprim = 0;
   uint32_t stack_depth = MAX_STACK;  // traversal stack bound of the scene's 4-wide nodes
   uint32_t lds_stack_cap = 0;        // 16, 24 or 32: LDS stack entries at most (0: MAX_STACK)
+  // TLAS refit of moved instances: per-object boxes, the TLAS nodes by level
+  // (deepest first) and the top copy's sources, all in scene_bufs
+  const float* obj_box = nullptr;
+  const uint32_t* tlas_order = nullptr;
+  const uint32_t* tlas_level_end = nullptr;
+  uint32_t tlas_levels = 0;
+  const uint32_t* top_src = nullptr;
+  double* refit_cost = nullptr;
+  double tlas_cost0 = 0;              // the TLAS's cost as built
+  std::vector<uint32_t> inst_object;  // object of each instance as uploaded
+  uint32_t ninst_sshapes = 0;
+  uint32_t n_uploads = 0, n_refits = 0;
   // solve buffers
   DevBuf start_hist;  // per-instance histogram / offsets of the first realisations
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, scan_chunks, counters,
@@ -2188,10 +2291,14 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   }
   r |= upload(c, objects.data(), objects.size(), &d.objects);
   r |= upload(c, col.out.data(), col.out.size(), &d.nodes);
+  // host arrays of the uploads below, alive until the closing synchronize
+  std::vector<Node4> top;
+  std::vector<uint32_t> top_order{troot}, tlas_order, level_end;
+  static const double zero = 0;
   {
     // the TLAS's first TOP_NODES nodes, breadth first, with the references
     // between them rewritten to TOP_REF | index
-    std::vector<uint32_t> order{troot};
+    std::vector<uint32_t>& order = top_order;
     for (size_t h = 0; h < order.size() && order.size() < (size_t)TOP_NODES; ++h)
       for (int k = 0; k < 4 && order.size() < (size_t)TOP_NODES; ++k) {
         const uint32_t ch = col.out[order[h]].child[k];
@@ -2199,7 +2306,6 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
       }
     std::map<uint32_t, uint32_t> where;
     for (size_t i = 0; i < order.size(); ++i) where[order[i]] = (uint32_t)i;
-    std::vector<Node4> top;
     for (uint32_t g : order) {
       Node4 n = col.out[g];
       for (int k = 0; k < 4; ++k) {
@@ -2211,7 +2317,40 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
     r |= upload(c, top.data(), top.size(), &d.top);
     d.ntop = (uint32_t)top.size();
     d.root_ref = TOP_REF;
+    r |= upload(c, order.data(), order.size(), &c->top_src);
   }
+  {
+    // TLAS nodes by depth, deepest level first (the refit's order), and the
+    // tree's cost as built
+    std::vector<std::vector<uint32_t>> levels;
+    std::vector<std::pair<uint32_t, uint32_t>> todo{{troot, 0u}};
+    double cost = 0;
+    while (!todo.empty()) {
+      const auto [nd, depth] = todo.back();
+      todo.pop_back();
+      if (levels.size() <= depth) levels.resize(depth + 1);
+      levels[depth].push_back(nd);
+      const Node4& n = col.out[nd];
+      for (int k = 0; k < 4; ++k) {
+        if (n.child[k] == EMPTY4) continue;
+        const double x = (double)n.box[0][1][k] - n.box[0][0][k];
+        const double y = (double)n.box[1][1][k] - n.box[1][0][k];
+        const double z = (double)n.box[2][1][k] - n.box[2][0][k];
+        cost += x * y + y * z + z * x;
+        if (!(n.child[k] & LEAF)) todo.push_back({n.child[k], depth + 1});
+      }
+    }
+    for (size_t l = levels.size(); l-- > 0;) {
+      tlas_order.insert(tlas_order.end(), levels[l].begin(), levels[l].end());
+      level_end.push_back((uint32_t)tlas_order.size());
+    }
+    r |= upload(c, tlas_order.data(), tlas_order.size(), &c->tlas_order);
+    r |= upload(c, level_end.data(), level_end.size(), &c->tlas_level_end);
+    c->tlas_levels = (uint32_t)level_end.size();
+    c->tlas_cost0 = cost;
+    r |= upload(c, &zero, 1, (const double**)&c->refit_cost);
+  }
+  r |= upload(c, s->object_boxes, (size_t)s->nobjects * 6, &c->obj_box);
   // a lane's stack: the TLAS descent's pushes, the instance sentinel, the
   // deepest BLAS descent's pushes
   c->stack_depth = col.pb[troot] + 1 + blas_pb;
@@ -2236,8 +2375,58 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   c->nsamp = s->nsampled;
   c->nprim = s->nprim_tallies;
   c->ninstances = s->ninstances;
+  c->ninst_sshapes = s->ninst_sshapes;
+  c->inst_object.resize(s->ninstances);
+  for (uint32_t i = 0; i < s->ninstances; ++i) c->inst_object[i] = s->instances[i].object;
   HIP_TRY(hipStreamSynchronize(c->stream));  // the host arrays may go after return
   c->has_scene = true;
+  ++c->n_uploads;
+  return 0;
+}
+
+SSOLH_API int ssolh_scene_move_instances(ssolh_context* c, uint32_t ninstances,
+                                         const struct ssolh_instance* instances,
+                                         uint32_t ninst_sshapes,
+                                         const struct ssolh_inst_sshape* inst_sshapes,
+                                         double* cost_ratio) {
+  if (!c || !instances || (ninst_sshapes && !inst_sshapes)) {
+    set_err("ssolh_scene_move_instances: NULL argument");
+    return SSOLH_ERR_ARG;
+  }
+  if (!c->has_scene || ninstances != c->ninstances || ninst_sshapes != c->ninst_sshapes) {
+    set_err("ssolh_scene_move_instances: %u instances, %u (instance, shape) frames for a scene "
+            "of %u and %u", ninstances, ninst_sshapes, c->ninstances, c->ninst_sshapes);
+    return SSOLH_ERR_ARG;
+  }
+  for (uint32_t i = 0; i < ninstances; ++i)
+    if (instances[i].object != c->inst_object[i]) {
+      set_err("ssolh_scene_move_instances: instance %u changed object", i);
+      return SSOLH_ERR_ARG;
+    }
+  HIP_TRY(hipSetDevice(c->device));
+  DScene& d = c->scn;
+  HIP_TRY(hipMemcpyAsync((void*)d.insts, instances, ninstances * sizeof(ssolh_instance),
+                         hipMemcpyHostToDevice, c->stream));
+  if (ninst_sshapes)
+    HIP_TRY(hipMemcpyAsync((void*)d.isshapes, inst_sshapes,
+                           ninst_sshapes * sizeof(ssolh_inst_sshape), hipMemcpyHostToDevice,
+                           c->stream));
+  k_tlas_refit<<<1, REFIT_THREADS, 0, c->stream>>>(
+      d.insts, d.tlas_inst, c->obj_box, (Node4*)d.nodes, c->tlas_order, c->tlas_level_end,
+      c->tlas_levels, (Node4*)d.top, c->top_src, d.ntop, c->refit_cost);
+  HIP_TRY(hipGetLastError());
+  double cost = 0;
+  HIP_TRY(hipMemcpyAsync(&cost, c->refit_cost, sizeof(cost), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (cost_ratio) *cost_ratio = c->tlas_cost0 > 0 ? cost / c->tlas_cost0 : 1.0;
+  ++c->n_refits;
+  return 0;
+}
+
+SSOLH_API int ssolh_test_scene_counters(ssolh_context* c, uint32_t* uploads, uint32_t* refits) {
+  if (!c) return SSOLH_ERR_ARG;
+  if (uploads) *uploads = c->n_uploads;
+  if (refits) *refits = c->n_refits;
   return 0;
 }
 

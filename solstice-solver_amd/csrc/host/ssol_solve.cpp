@@ -22,6 +22,10 @@ using namespace ssol;
 
 namespace {
 
+// A TLAS refit for moved instances is kept while its summed child-box
+// surface area stays within this factor of the tree's as built
+constexpr double REFIT_LOOSENESS = 1.5;
+
 // deepest TLAS + BLAS descent accepted: the GPU traversal keeps its first
 // stack entries in LDS and the rest, for deep trees, in device memory
 constexpr int MAX_BVH_LEVELS = 200;
@@ -638,16 +642,16 @@ res_T build_tlas(ssol_device* dev, Flat& f) {
   return RES_OK;
 }
 
-// Instances moved, nothing else changed: new instance frames and a new TLAS
-// over the same objects, sampling tables and tally layout (the sampled
-// areas are object-space areas, which rigid motions keep).
-res_T refresh_transforms(ssol_scene* scn, Flat& f) {
+// Instances moved, nothing else changed: new instance frames over the same
+// objects, sampling tables and tally layout (the sampled areas are
+// object-space areas, which rigid motions keep).
+res_T refresh_frames(ssol_scene* scn, Flat& f) {
   if (f.instances.size() != scn->instances.size()) return RES_BAD_ARG;
   for (size_t i = 0; i < scn->instances.size(); ++i) {
     ssolh_instance& d = f.instances[i];
     instance_frames(scn->instances[i], d, f.isshapes.data() + d.isshape_first);
   }
-  return build_tlas(scn->dev, f);
+  return RES_OK;
 }
 
 res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
@@ -766,6 +770,8 @@ res_T flatten(ssol_scene* scn, Flat& f, Layout& L) {
   s.sshapes = f.sshapes.data();
   s.nobjects = (uint32_t)f.objects.size();
   s.objects = f.objects.data();
+  static_assert(sizeof(aabb) == 6 * sizeof(float), "aabb is lo xyz, hi xyz");
+  s.object_boxes = (const float*)f.obj_box.data();
   s.ntriangles = (uint32_t)f.tris.size();
   s.triangles = f.tris.data();
   s.ninstances = (uint32_t)f.instances.size();
@@ -841,8 +847,22 @@ res_T prepare_scene(ssol_scene* scn) {
       return r;
     }
   } else if (!cached) {
-    // only instances moved: keep the objects, rebuild frames and the TLAS
-    res_T r = refresh_transforms(scn, dev->cache->flat);
+    // only instances moved: new frames, and the TLAS refit on the GPU while
+    // it stays within REFIT_LOOSENESS of the tree as built; else a new TLAS
+    // and a full upload
+    Flat& f = dev->cache->flat;
+    res_T r = refresh_frames(scn, f);
+    if (r == RES_OK && dev->gpu) {
+      double ratio = 0;
+      const int e = ssolh_scene_move_instances(dev->gpu, (uint32_t)f.instances.size(),
+                                               f.instances.data(), (uint32_t)f.isshapes.size(),
+                                               f.isshapes.data(), &ratio);
+      if (!e && ratio <= REFIT_LOOSENESS) {
+        dev->cache->xf.swap(xf);
+        return RES_OK;
+      }
+    }
+    if (r == RES_OK) r = build_tlas(dev, f);
     if (r != RES_OK) {
       solve_cache_free(dev);
       return r;
