@@ -407,6 +407,8 @@ def main(argv=None):
         kernel, ms, per_seg = "k_shade", shade_ms, (BYTES_SHADE_FP32 if fp32 else BYTES_SHADE_FP64)
     achieved = per_seg * segs / (ms * 1e-3) / 1e9
     seg_per_launch = segs / iters
+    if fp32:
+        kernel += "32"   # the FP32 form's kernels: k_trace32, k_shade32
     tr = load_traffic(kernel, workload + ("_fp32" if fp32 else ""))
     traffic = None if tr is None else round(tr["hbm_bytes_per_segment"] * seg_per_launch)
     roofline = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
