@@ -503,8 +503,8 @@ __device__ __forceinline__ double ray_triangle(d3 o, const Shear& s, d3 v0, d3 v
 
 struct TraceState {
   double tbest;
-  double qdst;
-  d3 qn;
+  double qdst;     // nearest refined quadric distance
+  uint32_t qiss;   // its (instance, shaded shape): the normal is rebuilt after the traversal
   uint32_t tri;  // the best triangle (its world normal is rebuilt after the traversal)
   uint32_t iss, prim;
   bool punched;  // the best triangle belongs to a punched surface
@@ -553,9 +553,20 @@ __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t is
   }
   if (punched && ts.qdst >= dst) {
     ts.qdst = dst;
-    ts.qn = qn;
+    ts.qiss = iss_id;
   }
   return true;
+}
+
+// Unit world normal of the quadric of (instance, shaded shape) `iss` where the
+// world ray (o, d) meets it at distance dst: the gradient of quadric_hit's
+// refinement at the same point, as the filter took it
+__device__ d3 quadric_normal(const DScene& S, uint32_t iss, d3 o, d3 d, double dst) {
+  const ssolh_inst_sshape& q = S.isshapes[iss];
+  const ssolh_shape& sh = S.shapes[S.iss[iss].shape];
+  const d3 lo = mul(q.R_inv, o - ld3(q.T));
+  const d3 ldir = mul(q.R_inv, d);
+  return normalize(tmul(q.R_inv, quadric_grad(sh.quadric, sh.qparam, lo + ldir * dst)));
 }
 
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -695,7 +706,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
   TraceState ts;
   ts.tbest = INFINITY;
   ts.qdst = INFINITY;
-  ts.qn = mk(0, 0, 0);
+  ts.qiss = NONE;
   ts.tri = 0;
   ts.iss = NONE;
   ts.prim = 0;
@@ -792,7 +803,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
   h.qdst = ts.qdst;
   d3 n = mk(0, 0, 0);
   if (ts.punched)
-    n = ts.qn;
+    n = quadric_normal(S, ts.qiss, ow, dw, ts.qdst);
   else if (ts.iss != NONE)
     n = mesh_ng(S, S.iss[ts.iss].inst, S.tris[ts.tri]);
   h.nx = n.x; h.ny = n.y; h.nz = n.z;
