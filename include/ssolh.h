@@ -474,6 +474,14 @@ SSOLH_API int ssolh_test_wave_sum
  * scene's bound (0 without a scene). The results do not depend on the cap. */
 SSOLH_API int ssolh_test_trace_stack(ssolh_context* ctx, uint32_t lds_cap, uint32_t* bound);
 
+/* Slack of the trace kernels' box tests, per unit of the largest |origin /
+ * direction| component of the ray (default 2^-21, which bounds the
+ * rounding of the FMA plane distances with margin): the scale for this
+ * context's current and later scenes, in [0, 1e-3]. Results do not change
+ * with any scale at or above the default; 0 lets rounding cull boxes that
+ * rays cross near their faces. */
+SSOLH_API int ssolh_test_slab_slack(ssolh_context* ctx, float scale);
+
 /* Scene uploads and TLAS refits this context has done */
 SSOLH_API int ssolh_test_scene_counters(ssolh_context* ctx, uint32_t* uploads, uint32_t* refits);
 

@@ -57,6 +57,7 @@ constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t TOP_REF = 0x40000000u;  // inner-node reference into the LDS copy
 constexpr uint32_t EMPTY4 = 0xFFFFFFFEu;   // absent child of a 4-wide node
 constexpr int TOP_NODES = 31;              // TLAS nodes staged in LDS (3.9 KB): 3 levels
+constexpr float SLAB_SLACK = 0x1p-21f;     // box-test slack (FRay), per unit of max |o*|
 constexpr int MAX_PEND = 8;          // compile-time bound of pending entries
 
 // Per-thread message of the last failing call
@@ -125,6 +126,7 @@ struct DScene {
   const Node4* top;
   uint32_t ntop;
   uint32_t root_ref;         // first node reference of a traversal
+  float slack_scale;         // box-test slack per unit of max |o*| (SLAB_SLACK)
   const ssolh_triangle* tris;
   const ssolh_instance* insts;
   const uint32_t* tlas_inst;
@@ -421,9 +423,14 @@ __device__ void block_count(bool pred, uint32_t* out, uint32_t b) {
 // -------------------------------------------------------------------------
 // Ray of the box tests: the distance to the plane x = p is p * ix + ox (one
 // FMA), and the rows of a node that hold the near planes of the ray's
-// direction signs are read by offset, so a slab needs no min/max. ox, oy, oz
-// are kept finite; a ray with a non-finite origin or direction gets
-// ox = oy = oz = +inf and crosses no box.
+// direction signs are read by offset, so a slab needs no min/max. The
+// rounding of ox = -o * ix puts up to |ox| 2^-24 of error on every plane
+// distance, which with km-scale origins is far more than the (p - o) * ix
+// form's error near the origin (about 6e-5 m against 1e-6 m at 1.5 km):
+// a box is crossed when tnear - slack <= tfar, slack = 2^-21 max |o*|
+// (SLAB_SLACK), so the test culls no box the subtracting form keeps. ox, oy, oz are kept
+// finite; a ray with a non-finite origin or direction gets
+// ox = oy = oz = +inf, so tnear - slack is NaN and it crosses no box.
 struct FRay {
   float ix, iy, iz;      // 1 / direction (components at least 1e-30 in magnitude)
   float ox, oy, oz;      // -origin / direction
@@ -651,15 +658,16 @@ __device__ __forceinline__ bool tpop(TStack& s, uint32_t& ref) {
 // child crossed, `ref` is popped; false when the stack is empty.
 template <int STACK>
 __device__ __forceinline__ bool visit4(const NodeRows& w, const FRay& fr, float tmin, float tmax,
-                                       uint32_t& ref, TStack& st) {
+                                       float slack_scale, uint32_t& ref, TStack& st) {
   float k[4];
   uint32_t r[4];
+  const float slack = slack_scale * fmaxf(fabsf(fr.ox), fmaxf(fabsf(fr.oy), fabsf(fr.oz)));
   for (int c = 0; c < 4; ++c) {
     const float tn = fmaxf(fmaxf(fmaf(w.n[0][c], fr.ix, fr.ox), fmaf(w.n[1][c], fr.iy, fr.oy)),
                            fmaxf(fmaf(w.n[2][c], fr.iz, fr.oz), tmin));
     const float tf = fminf(fminf(fmaf(w.f[0][c], fr.ix, fr.ox), fmaf(w.f[1][c], fr.iy, fr.oy)),
                            fminf(fmaf(w.f[2][c], fr.iz, fr.oz), tmax));
-    k[c] = tn <= tf ? tn : INFINITY;
+    k[c] = tn - slack <= tf ? tn : INFINITY;
     r[c] = w.child[c];
   }
   // sorting network (0,1)(2,3)(0,2)(1,3)(1,2)
@@ -741,7 +749,7 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
       // through its own address space), then its four boxes
       const NodeRows n =
           (ref & TOP_REF) ? lds_rows(lds_top, ref & 0x0FFFFFFFu, fr) : node_rows(S.nodes, ref, fr);
-      if (!visit4<STACK>(n, fr, ftmin, tmax, ref, st)) {
+      if (!visit4<STACK>(n, fr, ftmin, tmax, S.slack_scale, ref, st)) {
         alive = false;
         break;
       }
@@ -1897,6 +1905,7 @@ struct ssolh_context {
   uint64_t nprim = 0;
   uint32_t stack_depth = MAX_STACK;  // traversal stack bound of the scene's 4-wide nodes
   uint32_t lds_stack_cap = 0;        // 16, 24 or 32: LDS stack entries at most (0: MAX_STACK)
+  float slack_scale = SLAB_SLACK;     // box-test slack of the scenes uploaded next
   // TLAS refit of moved instances: per-object boxes, the TLAS nodes by level
   // (deepest first) and the top copy's sources, all in scene_bufs
   const float* obj_box = nullptr;
@@ -2328,6 +2337,7 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
     r |= upload(c, top.data(), top.size(), &d.top);
     d.ntop = (uint32_t)top.size();
     d.root_ref = TOP_REF;
+    d.slack_scale = c->slack_scale;
     r |= upload(c, order.data(), order.size(), &c->top_src);
   }
   {
@@ -2431,6 +2441,16 @@ SSOLH_API int ssolh_scene_move_instances(ssolh_context* c, uint32_t ninstances,
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (cost_ratio) *cost_ratio = c->tlas_cost0 > 0 ? cost / c->tlas_cost0 : 1.0;
   ++c->n_refits;
+  return 0;
+}
+
+SSOLH_API int ssolh_test_slab_slack(ssolh_context* c, float scale) {
+  if (!c || !(scale >= 0.f && scale <= 1e-3f)) {
+    set_err("ssolh_test_slab_slack: scale outside [0, 1e-3]");
+    return SSOLH_ERR_ARG;
+  }
+  c->slack_scale = scale;
+  c->scn.slack_scale = c->slack_scale;
   return 0;
 }
 

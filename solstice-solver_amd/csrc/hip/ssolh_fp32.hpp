@@ -208,7 +208,7 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
     while (!(ref & LEAF)) {
       const NodeRows n =
           (ref & TOP_REF) ? lds_rows(lds_top, ref & 0x0FFFFFFFu, fr) : node_rows(S.nodes, ref, fr);
-      if (!visit4<STACK>(n, fr, ftmin, tmax, ref, st)) {
+      if (!visit4<STACK>(n, fr, ftmin, tmax, S.slack_scale, ref, st)) {
         alive = false;
         break;
       }
