@@ -1,7 +1,8 @@
 """Moving instances of a scene that has already been solved
-(ssol_instance_set_transform between solves): the next solve rebuilds only
-the instance frames and the top-level hierarchy, and must give what a scene
-built anew with the new transforms gives (same realisations; the FP64 tally
+(ssol_instance_set_transform between solves): the next solve takes new
+instance frames and refits the top-level hierarchy on the GPU (rebuilding
+it when the refit gets loose), and must give what a scene built anew with
+the new transforms gives (same realisations; the FP64 tally
 additions may come in another order, tolerance 1e-10 relative)."""
 import numpy as np
 import pytest
