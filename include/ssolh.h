@@ -348,6 +348,14 @@ SSOLH_API int ssolh_scene_move_instances
    uint32_t ninst_sshapes, const struct ssolh_inst_sshape* inst_sshapes,
    double* cost_ratio);
 
+/* Builds the TLAS anew on the GPU over the instances' current world boxes
+ * (linear BVH over Morton codes of the box centres, collapsed to 4-wide
+ * nodes), in place of the uploaded one: for instances that moved too far
+ * for a refit (reference: the views rebuilt at every solve,
+ * src/ssol_scene.c:346-433). Uses the instance records the last upload or
+ * ssolh_scene_move_instances left on the device; returns after the build. */
+SSOLH_API int ssolh_scene_build_tlas(ssolh_context* ctx);
+
 /* Run the realisations of one solve over the uploaded scene */
 SSOLH_API int ssolh_solve(ssolh_context* ctx, const struct ssolh_solve_args* args);
 
@@ -482,8 +490,9 @@ SSOLH_API int ssolh_test_trace_stack(ssolh_context* ctx, uint32_t lds_cap, uint3
  * rays cross near their faces. */
 SSOLH_API int ssolh_test_slab_slack(ssolh_context* ctx, float scale);
 
-/* Scene uploads and TLAS refits this context has done */
-SSOLH_API int ssolh_test_scene_counters(ssolh_context* ctx, uint32_t* uploads, uint32_t* refits);
+/* Scene uploads, TLAS refits and GPU TLAS builds this context has done */
+SSOLH_API int ssolh_test_scene_counters(ssolh_context* ctx, uint32_t* uploads, uint32_t* refits,
+                                        uint32_t* builds);
 
 /* Bandwidth of a plain device copy of `bytes` bytes repeated `reps` times
  * (read + write bytes over the HIP-event time), in GB/s. Calibrates the HBM

@@ -27,6 +27,7 @@
 //
 // The host loop runs trace / shade / scan / scatter until the list is empty.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <float.h>
 #include <math.h>
@@ -933,6 +934,308 @@ k_tlas_refit(const ssolh_instance* __restrict__ insts, const uint32_t* __restric
     __syncthreads();
   }
   if (threadIdx.x == 0) *cost = part[0];
+}
+
+// -------------------------------------------------------------------------
+// TLAS build on the GPU (linear BVH, T. Karras, "Maximizing parallelism in
+// the construction of BVHs, octrees, and k-d trees", HPG 2012): when moved
+// instances have made a refit tree too loose, the TLAS is built anew over
+// the instances' new world boxes without the host. Keys: 30-bit Morton codes
+// of the box centres in the centres' bounds (ties broken by sorted position);
+// a radix sort of (key, instance); the binary radix tree; boxes bottom-up;
+// every BVH2 node at even depth becomes a 4-wide node over its children and
+// grandchildren. Two single-workgroup kernels around the sort: the TLAS of a
+// field holds tens of thousands of instances at most, which one workgroup
+// handles in microseconds.
+// -------------------------------------------------------------------------
+constexpr int LBVH_THREADS = 1024;
+constexpr int LBVH_MAX_LEVELS = 66;  // 4-wide levels of a tree of 30-bit keys + 32-bit ties
+
+struct LbvhResult {
+  uint32_t nodes4;  // 4-wide nodes written
+  uint32_t depth4;  // deepest 4-wide level (root 0)
+  uint32_t ntop;    // nodes of the LDS copy
+  uint32_t pad;
+  double cost;      // summed child-box surface area
+};
+
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+  x &= 0x3FFu;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+__global__ void __launch_bounds__(LBVH_THREADS)
+k_lbvh_keys(const ssolh_instance* __restrict__ insts, const float* __restrict__ obj_box,
+            uint32_t n, float* __restrict__ lbox, uint32_t* __restrict__ key,
+            uint32_t* __restrict__ val) {
+  __shared__ float red[6][LBVH_THREADS];
+  float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (uint32_t i = threadIdx.x; i < n; i += LBVH_THREADS) {
+    float lo[3], hi[3];
+    instance_world_box(insts[i], obj_box + 6 * (size_t)insts[i].object, lo, hi);
+    for (int a = 0; a < 3; ++a) {
+      lbox[6 * (size_t)i + a] = lo[a];
+      lbox[6 * (size_t)i + 3 + a] = hi[a];
+      const float cen = 0.5f * (lo[a] + hi[a]);
+      mn[a] = fminf(mn[a], cen);
+      mx[a] = fmaxf(mx[a], cen);
+    }
+  }
+  for (int a = 0; a < 3; ++a) { red[a][threadIdx.x] = mn[a]; red[3 + a][threadIdx.x] = mx[a]; }
+  __syncthreads();
+  for (uint32_t w = LBVH_THREADS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int a = 0; a < 3; ++a) {
+        red[a][threadIdx.x] = fminf(red[a][threadIdx.x], red[a][threadIdx.x + w]);
+        red[3 + a][threadIdx.x] = fmaxf(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  float lo[3], sc[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = red[a][0];
+    const float ext = red[3 + a][0] - red[a][0];
+    sc[a] = ext > 0.f ? 1023.f / ext : 0.f;
+  }
+  for (uint32_t i = threadIdx.x; i < n; i += LBVH_THREADS) {
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a) {
+      const float cen = 0.5f * (lbox[6 * (size_t)i + a] + lbox[6 * (size_t)i + 3 + a]);
+      q[a] = (uint32_t)fminf(fmaxf((cen - lo[a]) * sc[a], 0.f), 1023.f);
+    }
+    key[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    val[i] = i;
+  }
+}
+
+// Common-prefix length of sorted keys i and j (-1 out of range; equal keys
+// compare their positions)
+__device__ __forceinline__ int lbvh_delta(const uint32_t* k, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = k[i], b = k[j];
+  return a == b ? 32 + __clz(i ^ j) : __clz(a ^ b);
+}
+
+// Work arrays of the build: child pairs and parents of the binary tree
+// (internal nodes 0..n-2, leaf i as n-1+i), arrival counters, internal
+// boxes, depths, 4-wide indices and levels
+struct LbvhWork {
+  int2* child;
+  int* parent;
+  unsigned* flag;
+  float* ibox;
+  uint32_t* depth;
+  uint32_t* idx4;
+  uint32_t* dep4;
+};
+
+__device__ __forceinline__ void lbvh_box(const LbvhWork& w, const float* lbox,
+                                         const uint32_t* sval, int n, int x, float b[6]) {
+  const float* src = x < n - 1 ? w.ibox + 6 * (size_t)x : lbox + 6 * (size_t)sval[x - (n - 1)];
+  for (int k = 0; k < 6; ++k) b[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void __launch_bounds__(LBVH_THREADS)
+k_lbvh_finish(uint32_t n, const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval,
+              const float* __restrict__ lbox, LbvhWork w, Node4* tlas, uint32_t tlas_base,
+              uint32_t* tlas_inst, uint32_t* order, uint32_t* level_end, Node4* top,
+              uint32_t* top_src, LbvhResult* res) {
+  __shared__ uint32_t s_scan[LBVH_THREADS];
+  __shared__ uint32_t s_lvl[LBVH_MAX_LEVELS];
+  __shared__ uint32_t s_cur[LBVH_MAX_LEVELS];
+  __shared__ double s_cost[LBVH_THREADS];
+  __shared__ uint32_t s_max;
+  const int N = (int)n;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < n; i += LBVH_THREADS) tlas_inst[i] = sval[i];
+  if (N == 1) {
+    // one instance: a root with one leaf
+    if (tid == 0) {
+      float b[6];
+      for (int k = 0; k < 6; ++k) b[k] = lbox[6 * (size_t)sval[0] + k];
+      Node4 o;
+      for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 4; ++c) {
+          o.box[a][0][c] = c == 0 ? b[a] : INFINITY;
+          o.box[a][1][c] = c == 0 ? b[3 + a] : -INFINITY;
+        }
+      for (int c = 0; c < 4; ++c) { o.child[c] = c == 0 ? LEAF : EMPTY4; o.pad[c] = 0; }
+      tlas[0] = o;
+      top[0] = o;
+      top_src[0] = tlas_base;
+      order[0] = tlas_base;
+      level_end[0] = 1;
+      const double x = (double)b[3] - b[0], y = (double)b[4] - b[1], z = (double)b[5] - b[2];
+      *res = LbvhResult{1u, 0u, 1u, 0u, x * y + y * z + z * x};
+    }
+    return;
+  }
+  // binary radix tree
+  for (uint32_t t = tid; t < n - 1; t += LBVH_THREADS) {
+    const int i = (int)t;
+    w.flag[i] = 0;
+    const int d = lbvh_delta(skey, N, i, i + 1) - lbvh_delta(skey, N, i, i - 1) >= 0 ? 1 : -1;
+    const int dmin = lbvh_delta(skey, N, i, i - d);
+    int lmax = 2;
+    while (lbvh_delta(skey, N, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int s = lmax / 2; s >= 1; s /= 2)
+      if (lbvh_delta(skey, N, i, i + (l + s) * d) > dmin) l += s;
+    const int j = i + l * d;
+    const int dnode = lbvh_delta(skey, N, i, j);
+    int sp = 0;
+    for (int s = (l + 1) / 2;; s = (s + 1) / 2) {
+      if (lbvh_delta(skey, N, i, i + (sp + s) * d) > dnode) sp += s;
+      if (s == 1) break;
+    }
+    const int g = i + sp * d + min(d, 0);
+    const int left = min(i, j) == g ? N - 1 + g : g;
+    const int right = max(i, j) == g + 1 ? N - 1 + g + 1 : g + 1;
+    w.child[i] = make_int2(left, right);
+    w.parent[left] = i;
+    w.parent[right] = i;
+  }
+  if (tid == 0) w.parent[0] = -1;
+  __threadfence();
+  __syncthreads();
+  // boxes bottom-up: the second of two children to arrive at a node forms it
+  for (uint32_t t = tid; t < n; t += LBVH_THREADS) {
+    int node = w.parent[N - 1 + (int)t];
+    while (node >= 0) {
+      const unsigned prev =
+          __hip_atomic_fetch_add(w.flag + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (prev == 0) break;
+      float a[6], b[6];
+      lbvh_box(w, lbox, sval, N, w.child[node].x, a);
+      lbvh_box(w, lbox, sval, N, w.child[node].y, b);
+      float* dst = w.ibox + 6 * (size_t)node;
+      for (int k = 0; k < 3; ++k) {
+        __hip_atomic_store(dst + k, fminf(a[k], b[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(dst + 3 + k, fmaxf(a[3 + k], b[3 + k]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+      node = w.parent[node];
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  // depths, then the 4-wide index of every even-depth node (block scan)
+  for (uint32_t t = tid; t < n - 1; t += LBVH_THREADS) {
+    uint32_t dd = 0;
+    for (int p = w.parent[t]; p >= 0; p = w.parent[p]) ++dd;
+    w.depth[t] = dd;
+  }
+  if (tid == 0) s_max = 0;
+  __threadfence();
+  __syncthreads();
+  uint32_t run = 0;
+  for (uint32_t base = 0; base < n - 1; base += LBVH_THREADS) {
+    const uint32_t t = base + tid;
+    const uint32_t v = (t < n - 1 && !(w.depth[t] & 1u)) ? 1u : 0u;
+    s_scan[tid] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < LBVH_THREADS; off <<= 1) {
+      const uint32_t x = tid >= off ? s_scan[tid - off] : 0u;
+      __syncthreads();
+      s_scan[tid] += x;
+      __syncthreads();
+    }
+    if (t < n - 1) w.idx4[t] = run + s_scan[tid] - v;
+    run += s_scan[LBVH_THREADS - 1];
+    __syncthreads();
+  }
+  __threadfence();
+  __syncthreads();
+  // the 4-wide nodes
+  double acc = 0;
+  for (uint32_t t = tid; t < n - 1; t += LBVH_THREADS) {
+    if (w.depth[t] & 1u) continue;
+    int ent[4], ne = 0;
+    const int2 c2 = w.child[t];
+    for (int side = 0; side < 2; ++side) {
+      const int c = side ? c2.y : c2.x;
+      if (c >= N - 1) {
+        ent[ne++] = c;
+      } else {
+        ent[ne++] = w.child[c].x;
+        ent[ne++] = w.child[c].y;
+      }
+    }
+    Node4 o;
+    for (int e = 0; e < 4; ++e) {
+      if (e >= ne) {
+        for (int a = 0; a < 3; ++a) { o.box[a][0][e] = INFINITY; o.box[a][1][e] = -INFINITY; }
+        o.child[e] = EMPTY4;
+        o.pad[e] = 0;
+        continue;
+      }
+      float b[6];
+      lbvh_box(w, lbox, sval, N, ent[e], b);
+      for (int a = 0; a < 3; ++a) { o.box[a][0][e] = b[a]; o.box[a][1][e] = b[3 + a]; }
+      o.child[e] = ent[e] >= N - 1 ? (LEAF | (uint32_t)(ent[e] - (N - 1)))
+                                   : tlas_base + w.idx4[ent[e]];
+      o.pad[e] = 0;
+      const double x = (double)b[3] - b[0], y = (double)b[4] - b[1], z = (double)b[5] - b[2];
+      acc += x * y + y * z + z * x;
+    }
+    const uint32_t k = w.idx4[t];
+    tlas[k] = o;
+    const uint32_t d4 = w.depth[t] / 2;
+    w.dep4[k] = d4;
+    atomicMax(&s_max, d4);
+  }
+  s_cost[tid] = acc;
+  __threadfence();
+  __syncthreads();
+  for (uint32_t s = LBVH_THREADS / 2; s > 0; s >>= 1) {
+    if (tid < s) s_cost[tid] += s_cost[tid + s];
+    __syncthreads();
+  }
+  // levels for the refit, deepest first
+  const uint32_t n4 = run, dmax = s_max;
+  for (uint32_t l = tid; l < LBVH_MAX_LEVELS; l += LBVH_THREADS) s_lvl[l] = 0;
+  __syncthreads();
+  for (uint32_t k = tid; k < n4; k += LBVH_THREADS) atomicAdd(&s_lvl[w.dep4[k]], 1u);
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t acc_l = 0;
+    for (int l = (int)dmax; l >= 0; --l) {
+      s_cur[l] = acc_l;
+      acc_l += s_lvl[l];
+      level_end[dmax - l] = acc_l;
+    }
+  }
+  __syncthreads();
+  for (uint32_t k = tid; k < n4; k += LBVH_THREADS)
+    order[atomicAdd(&s_cur[w.dep4[k]], 1u)] = tlas_base + k;
+  __threadfence();
+  __syncthreads();
+  // the LDS copy of the top levels, breadth first
+  if (tid == 0) {
+    uint32_t q[TOP_NODES];
+    uint32_t nq = 1;
+    q[0] = tlas_base;
+    for (uint32_t h = 0; h < nq && nq < (uint32_t)TOP_NODES; ++h)
+      for (int c = 0; c < 4 && nq < (uint32_t)TOP_NODES; ++c) {
+        const uint32_t ch = tlas[q[h] - tlas_base].child[c];
+        if (!(ch & LEAF)) q[nq++] = ch;
+      }
+    for (uint32_t h = 0; h < nq; ++h) {
+      Node4 o = tlas[q[h] - tlas_base];
+      for (int c = 0; c < 4; ++c) {
+        if (o.child[c] & LEAF) continue;
+        for (uint32_t g = 0; g < nq; ++g)
+          if (q[g] == o.child[c]) { o.child[c] = TOP_REF | g; break; }
+      }
+      top[h] = o;
+      top_src[h] = q[h];
+    }
+    *res = LbvhResult{n4, dmax, nq, 0u, s_cost[0]};
+  }
 }
 
 // -------------------------------------------------------------------------
@@ -1917,7 +2220,11 @@ struct ssolh_context {
   double tlas_cost0 = 0;              // the TLAS's cost as built
   std::vector<uint32_t> inst_object;  // object of each instance as uploaded
   uint32_t ninst_sshapes = 0;
-  uint32_t n_uploads = 0, n_refits = 0;
+  uint32_t n_uploads = 0, n_refits = 0, n_builds = 0;
+  // GPU TLAS builds: the TLAS's region of `nodes` (first index, capacity),
+  // the BLASes' part of the stack bound, the build's work buffer
+  uint32_t tlas_base = 0, tlas_cap = 0, blas_pb = 0;
+  DevBuf lbvh;
   // solve buffers
   DevBuf start_hist;  // per-instance histogram / offsets of the first realisations
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, scan_chunks, counters,
@@ -2245,7 +2552,7 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
   free_scene(c);
   for (DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
                     &c->block_emits, &c->block_offsets, &c->scan_chunks, &c->counters, &c->tallies,
-                    &c->jobs, &c->start_hist, &c->trace_ovf,
+                    &c->jobs, &c->start_hist, &c->trace_ovf, &c->lbvh,
                     &c->adj_state, &c->adj_in, &c->adj_grad, &c->trk_vtx, &c->trk_type,
                     &c->trk_seq, &c->trk_count, &c->trk_pos})
     if (b->p) (void)hipFree(b->p);
@@ -2310,6 +2617,11 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
     return SSOLH_ERR_LIMIT;
   }
   r |= upload(c, objects.data(), objects.size(), &d.objects);
+  // the TLAS region holds any tree over the instances (a GPU build writes
+  // at most ninstances 4-wide nodes)
+  c->tlas_base = troot;
+  c->tlas_cap = std::max<uint32_t>((uint32_t)col.out.size() - troot, s->ninstances);
+  col.out.resize((size_t)troot + c->tlas_cap);
   r |= upload(c, col.out.data(), col.out.size(), &d.nodes);
   // host arrays of the uploads below, alive until the closing synchronize
   std::vector<Node4> top;
@@ -2334,10 +2646,12 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
       }
       top.push_back(n);
     }
-    r |= upload(c, top.data(), top.size(), &d.top);
     d.ntop = (uint32_t)top.size();
     d.root_ref = TOP_REF;
     d.slack_scale = c->slack_scale;
+    top.resize(TOP_NODES);
+    r |= upload(c, top.data(), top.size(), &d.top);
+    order.resize(TOP_NODES);
     r |= upload(c, order.data(), order.size(), &c->top_src);
   }
   {
@@ -2365,9 +2679,11 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
       tlas_order.insert(tlas_order.end(), levels[l].begin(), levels[l].end());
       level_end.push_back((uint32_t)tlas_order.size());
     }
+    c->tlas_levels = (uint32_t)level_end.size();
+    tlas_order.resize(std::max<size_t>(tlas_order.size(), s->ninstances));
+    level_end.resize(std::max<size_t>(level_end.size(), LBVH_MAX_LEVELS));
     r |= upload(c, tlas_order.data(), tlas_order.size(), &c->tlas_order);
     r |= upload(c, level_end.data(), level_end.size(), &c->tlas_level_end);
-    c->tlas_levels = (uint32_t)level_end.size();
     c->tlas_cost0 = cost;
     r |= upload(c, &zero, 1, (const double**)&c->refit_cost);
   }
@@ -2375,6 +2691,7 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   // a lane's stack: the TLAS descent's pushes, the instance sentinel, the
   // deepest BLAS descent's pushes
   c->stack_depth = col.pb[troot] + 1 + blas_pb;
+  c->blas_pb = blas_pb;
   r |= upload(c, s->triangles, s->ntriangles, &d.tris);
   r |= upload(c, s->instances, s->ninstances, &d.insts);
   r |= upload(c, s->tlas_instances, s->ninstances, &d.tlas_inst);
@@ -2444,6 +2761,63 @@ SSOLH_API int ssolh_scene_move_instances(ssolh_context* c, uint32_t ninstances,
   return 0;
 }
 
+SSOLH_API int ssolh_scene_build_tlas(ssolh_context* c) {
+  if (!c || !c->has_scene) {
+    set_err("ssolh_scene_build_tlas: no scene");
+    return SSOLH_ERR_ARG;
+  }
+  const uint32_t n = c->ninstances;
+  HIP_TRY(hipSetDevice(c->device));
+  // work buffer: sort keys and values (twice), the radix sort's temporary,
+  // instance boxes, the binary tree, results
+  size_t sort_bytes = 0;
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint32_t*)nullptr,
+                                             (uint32_t*)nullptr, (const uint32_t*)nullptr,
+                                             (uint32_t*)nullptr, (int)n, 0, 30, c->stream));
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t nn = n, ni = n > 1 ? n - 1 : 1;
+  const size_t sizes[] = {nn * 4, nn * 4, nn * 4, nn * 4, sort_bytes, nn * 24, ni * 8,
+                          (2 * nn) * 4, ni * 4, ni * 24, ni * 4, ni * 4, nn * 4,
+                          sizeof(LbvhResult)};
+  size_t total = 0;
+  for (size_t b : sizes) total += al(b);
+  if (int e = dev_alloc(c->lbvh, total)) return e;
+  char* p = (char*)c->lbvh.p;
+  void* part[14];
+  for (int i = 0; i < 14; ++i) { part[i] = p; p += al(sizes[i]); }
+  uint32_t *key = (uint32_t*)part[0], *skey = (uint32_t*)part[1], *val = (uint32_t*)part[2],
+           *sval = (uint32_t*)part[3];
+  LbvhWork w{(int2*)part[6], (int*)part[7], (unsigned*)part[8], (float*)part[9],
+             (uint32_t*)part[10], (uint32_t*)part[11], (uint32_t*)part[12]};
+  LbvhResult* res = (LbvhResult*)part[13];
+  DScene& d = c->scn;
+  k_lbvh_keys<<<1, LBVH_THREADS, 0, c->stream>>>(d.insts, c->obj_box, n, (float*)part[5], key,
+                                                 val);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(part[4], sort_bytes, key, skey, val, sval, (int)n, 0,
+                                             30, c->stream));
+  k_lbvh_finish<<<1, LBVH_THREADS, 0, c->stream>>>(
+      n, skey, sval, (const float*)part[5], w, (Node4*)d.nodes + c->tlas_base, c->tlas_base,
+      (uint32_t*)d.tlas_inst, (uint32_t*)c->tlas_order, (uint32_t*)c->tlas_level_end,
+      (Node4*)d.top, (uint32_t*)c->top_src, res);
+  HIP_TRY(hipGetLastError());
+  LbvhResult h{};
+  HIP_TRY(hipMemcpyAsync(&h, res, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (h.nodes4 == 0 || h.nodes4 > c->tlas_cap || h.ntop == 0 || h.ntop > (uint32_t)TOP_NODES) {
+    set_err("ssolh_scene_build_tlas: inconsistent build (%u nodes, %u top)", h.nodes4, h.ntop);
+    return SSOLH_ERR_LIMIT;
+  }
+  d.ntop = h.ntop;
+  d.root_ref = TOP_REF;
+  c->tlas_levels = h.depth4 + 1;
+  c->tlas_cost0 = h.cost;
+  // a descent pushes at most 3 per 4-wide level above the leaves
+  c->stack_depth = 3u * (h.depth4 + 1u) + 1u + c->blas_pb;
+  ++c->n_builds;
+  return 0;
+}
+
 SSOLH_API int ssolh_test_slab_slack(ssolh_context* c, float scale) {
   if (!c || !(scale >= 0.f && scale <= 1e-3f)) {
     set_err("ssolh_test_slab_slack: scale outside [0, 1e-3]");
@@ -2454,10 +2828,12 @@ SSOLH_API int ssolh_test_slab_slack(ssolh_context* c, float scale) {
   return 0;
 }
 
-SSOLH_API int ssolh_test_scene_counters(ssolh_context* c, uint32_t* uploads, uint32_t* refits) {
+SSOLH_API int ssolh_test_scene_counters(ssolh_context* c, uint32_t* uploads, uint32_t* refits,
+                                        uint32_t* builds) {
   if (!c) return SSOLH_ERR_ARG;
   if (uploads) *uploads = c->n_uploads;
   if (refits) *refits = c->n_refits;
+  if (builds) *builds = c->n_builds;
   return 0;
 }
 

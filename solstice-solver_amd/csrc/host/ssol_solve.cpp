@@ -848,16 +848,17 @@ res_T prepare_scene(ssol_scene* scn) {
     }
   } else if (!cached) {
     // only instances moved: new frames, and the TLAS refit on the GPU while
-    // it stays within REFIT_LOOSENESS of the tree as built; else a new TLAS
-    // and a full upload
+    // it stays within REFIT_LOOSENESS of the tree as built, else built anew
+    // on the GPU; a full upload only if either fails
     Flat& f = dev->cache->flat;
     res_T r = refresh_frames(scn, f);
     if (r == RES_OK && dev->gpu) {
       double ratio = 0;
-      const int e = ssolh_scene_move_instances(dev->gpu, (uint32_t)f.instances.size(),
-                                               f.instances.data(), (uint32_t)f.isshapes.size(),
-                                               f.isshapes.data(), &ratio);
-      if (!e && ratio <= REFIT_LOOSENESS) {
+      int e = ssolh_scene_move_instances(dev->gpu, (uint32_t)f.instances.size(),
+                                         f.instances.data(), (uint32_t)f.isshapes.size(),
+                                         f.isshapes.data(), &ratio);
+      if (!e && ratio > REFIT_LOOSENESS) e = ssolh_scene_build_tlas(dev->gpu);
+      if (!e) {
         dev->cache->xf.swap(xf);
         return RES_OK;
       }

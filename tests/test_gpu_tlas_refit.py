@@ -1,5 +1,6 @@
 """Moved instances: the TLAS refit on the GPU (ssolh_scene_move_instances,
-reached through ssol_instance_set_transform and ssol_solve) against a scene
+reached through ssol_instance_set_transform and ssol_solve) and the TLAS
+built on the GPU (ssolh_scene_build_tlas, a linear BVH) against a scene
 built anew with the moved transforms (the reference rebuilds its views at
 every solve, src/ssol_scene.c:346-433).
 
@@ -9,8 +10,10 @@ fresh build: equal segment counts, expectations equal up to the order of
 the FP64 atomics (1e-10 relative; FP32 form 1e-6), variances and standard
 errors to 1e-6. Aiming steps (small
 rotations in place) stay on the refit path; a heliostat sent 5 km away
-loosens the tree past the bound, and the solve rebuilds and uploads the
-scene instead, with the same agreement."""
+loosens the tree past the bound, and the solve builds the TLAS anew on the
+GPU instead (no upload), with the same agreement. A GPU-built TLAS traces
+what the uploaded one traces on the 10k-heliostat field, on C1 and on a
+scene of one instance."""
 import ctypes as C
 
 import numpy as np
@@ -29,7 +32,9 @@ PATHS = 2_000_000
 def _hip():
     h = L.lib_hip()
     h.ssolh_test_scene_counters.argtypes = [C.c_void_p, C.POINTER(C.c_uint32),
-                                            C.POINTER(C.c_uint32)]
+                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    h.ssolh_scene_build_tlas.argtypes = [C.c_void_p]
+    h.ssolh_scene_build_tlas.restype = C.c_int
     h.ssolh_test_scene_counters.restype = C.c_int
     h.ssolh_scene_move_instances.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                                              C.c_void_p, C.POINTER(C.c_double)]
@@ -38,12 +43,16 @@ def _hip():
     return h
 
 
-def _counters(lib):
+def _ctx(lib):
     ctx = C.c_void_p()
     L.call("ssol_device_get_gpu_context", lib.dev, C.byref(ctx))
-    u, r = C.c_uint32(), C.c_uint32()
-    assert _hip().ssolh_test_scene_counters(ctx, C.byref(u), C.byref(r)) == 0
-    return u.value, r.value
+    return ctx
+
+
+def _counters(lib, builds=False):
+    u, r, b = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    assert _hip().ssolh_test_scene_counters(_ctx(lib), C.byref(u), C.byref(r), C.byref(b)) == 0
+    return (u.value, r.value, b.value) if builds else (u.value, r.value)
 
 
 def _field(aim):
@@ -117,9 +126,11 @@ def test_far_move_rebuilds(gpu):
     t[9] += 5000.0                          # 5 km east
     moved.instances[7].transform = t
     lib.set_transform(spec.instances[7], t)
+    b0 = _counters(lib, builds=True)[2]
     est = lib.solve(PATHS, seed=21)
-    up, rf = _counters(lib)
-    assert rf == rf0 + 1 and up == up0 + 1, (up, rf)  # refit tried, tree too loose: upload
+    up, rf, b = _counters(lib, builds=True)
+    # refit tried, tree too loose: built anew on the GPU, no upload
+    assert (up, rf, b) == (up0, rf0 + 1, b0 + 1), (up, rf, b)
     _same_as_fresh(spec, est, moved, "fp64", 21)
     # and moved back: refit again (the rebuilt tree is the new reference)
     lib.set_transform(spec.instances[7], spec.instances[7].transform)
@@ -139,4 +150,34 @@ def test_move_rejects_other_counts(gpu):
     r = hip.ssolh_scene_move_instances(ctx, 5, C.cast(buf, C.c_void_p), 0, None, None)
     assert r != 0 and b"instances" in hip.ssolh_last_error()
     assert hip.ssolh_scene_move_instances(ctx, N_HELIO + 1, None, 0, None, None) != 0
+    lib.close()
+
+
+def _one_instance():
+    plate = S.Object([(S.square_mesh(2.0), S.matte(0.0), S.matte(0.0))])
+    inst = S.Instance(plate, S.transform(None, (0.0, 0.0, 0.0)), receiver_mask=S.FRONT, sample=1)
+    sun = S.Sun("directional", S.sun_dir_from(180.0, 60.0), 1000.0, S.mono_spectrum())
+    return S.Scene([inst], sun, None, name="one_plate")
+
+
+@pytest.mark.parametrize("scene,precision", [("config3", "fp64"), ("config3", "fp32"),
+                                             ("config1", "fp64"), ("one", "fp64")])
+def test_gpu_built_tlas_traces_like_the_uploaded_one(gpu, scene, precision):
+    spec = {"config3": S.config3, "config1": S.config1, "one": _one_instance}[scene]()
+    lib = L.Scene(spec, precision=precision)
+    ref = lib.solve(PATHS, seed=5)
+    up0, rf0, b0 = _counters(lib, builds=True)
+    assert _hip().ssolh_scene_build_tlas(_ctx(lib)) == 0, _hip().ssolh_last_error()
+    est = lib.solve(PATHS, seed=5)
+    assert _counters(lib, builds=True) == (up0, rf0, b0 + 1)
+    assert est.stats.segments == ref.stats.segments
+    g, h = est.global_(), ref.global_()
+    rel = 1e-6 if precision == "fp32" else 1e-10
+    for k in g:
+        _close(np.array([g[k][0]]), np.array([h[k][0]]), rel)
+    if scene == "config3":
+        rcv = spec.instances[-1]
+        shape = rcv.object.sshapes[0][0]
+        _close(est.primitives(rcv, shape, S.FRONT, "absorbed_flux")[:, 0],
+               ref.primitives(rcv, shape, S.FRONT, "absorbed_flux")[:, 0], rel)
     lib.close()
