@@ -173,6 +173,16 @@ def sweep(n_suns=512, log_min=5.0, log_max=7.0, seq_suns=64, graph_iterations=16
         t_batch[graph] = time.perf_counter() - t0
         lib.close()
         settle()
+    # the widths and graph lengths picked by the autotuner on a probe batch
+    # (every 8th sun, a sixteenth of its realisations), then the sweep
+    lib = L.Scene(spec)
+    tuned = lib.autotune_batch(dirs[::8], [max(1000, int(c) // 16) for c in counts[::8]])
+    lib.solve_sun_batch(dirs[:4], [10000] * 4, seed=1)  # warm-up
+    t0 = time.perf_counter()
+    lib.solve_sun_batch(dirs, counts, seed=2)
+    t_tuned = time.perf_counter() - t0
+    lib.close()
+    settle()
     # sequential: one solve per sun position on one scene
     sl = L.Scene(spec)
     sl.solve(10000, seed=1)  # warm-up
@@ -196,7 +206,11 @@ def sweep(n_suns=512, log_min=5.0, log_max=7.0, seq_suns=64, graph_iterations=16
            "sequential_sample_s": round(t_seq, 4),
            "sequential_Mpaths_s": round(seq_rate, 2),
            "speedup_vs_sequential": round(rate[0] / seq_rate, 2),
-           "graph_speedup_vs_sequential": round(rate[graph_iterations] / seq_rate, 2)}
+           "graph_speedup_vs_sequential": round(rate[graph_iterations] / seq_rate, 2),
+           "autotuned": {"path_slots": tuned.path_slots,
+                         "graph_iterations": tuned.graph_iterations,
+                         "candidates": tuned.candidates, "batched_s": round(t_tuned, 4),
+                         "batched_Mpaths_s": round(n_total / t_tuned / 1e6, 2)}}
     print(f"[bench] sweep: {out}", file=sys.stderr, flush=True)
     return out
 

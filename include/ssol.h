@@ -695,6 +695,13 @@ struct ssol_autotune_result {
 SSOL_API res_T ssol_device_autotune
   (struct ssol_device* dev, struct ssol_scene* scn, const size_t probe,
    struct ssol_autotune_result* out);
+/* The same for sun batches: candidate widths and graph lengths timed on a
+ * probe run of ssol_solve_sun_batch over the given suns and realisation
+ * counts (a sample of the sweep to come). */
+SSOL_API res_T ssol_device_autotune_batch
+  (struct ssol_device* dev, struct ssol_scene* scn, const size_t nsuns,
+   struct ssol_sun* const* suns, const size_t* realisations,
+   struct ssol_autotune_result* out);
 
 #ifdef __cplusplus
 }

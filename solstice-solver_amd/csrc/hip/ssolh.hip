@@ -2200,6 +2200,10 @@ struct ssolh_context {
   hipStream_t stream = nullptr;
   void* scratch = nullptr;  // this device's copy of the scratch array
   hipEvent_t ev[4] = {};
+  // graph drain: the list length after each of two replays in flight, in
+  // pinned host memory, and the events that mark it copied
+  hipEvent_t gev[2] = {};
+  unsigned* pinned = nullptr;
   // scene
   std::vector<DevBuf> scene_bufs;
   DScene scn{};
@@ -2537,6 +2541,8 @@ SSOLH_API int ssolh_create(int device, ssolh_context** out) {
   c->scratch = scratch_ptr;
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
+  for (auto& e : c->gev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIP_TRY(hipHostMalloc((void**)&c->pinned, 2 * sizeof(unsigned), hipHostMallocDefault));
   if (int r = init_resident_grids(c, (uint32_t)prop.multiProcessorCount)) {
     ssolh_destroy(c);
     return r;
@@ -2557,6 +2563,8 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
                     &c->trk_seq, &c->trk_count, &c->trk_pos})
     if (b->p) (void)hipFree(b->p);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->gev) if (e) (void)hipEventDestroy(e);
+  if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -3321,22 +3329,42 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   const uint32_t gi = (graph_iters + 1) & ~1u;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // graph replays until the list is empty, two in flight: the list length
+  // after replay k is copied to pinned memory and read while replay k+1
+  // runs, so the GPU does not wait for the host between replays; a replay
+  // past the end runs empty launches
   auto drain_graph = [&](uint32_t count) -> int {
+    uint32_t launched = 0, seen = 0;
     while (count) {
-      if (wave_iter >= MAX_WAVE_ITER) {
-        set_err("ssolh_solve: the wavefront did not drain after %u iterations", wave_iter);
-        return SSOLH_ERR_LIMIT;
+      while (launched - seen < 2) {
+        if (wave_iter >= MAX_WAVE_ITER) {
+          set_err("ssolh_solve: the wavefront did not drain after %u iterations", wave_iter);
+          (void)hipStreamSynchronize(c->stream);
+          return SSOLH_ERR_LIMIT;
+        }
+        const uint32_t k = launched & 1u;
+        hipError_t e = hipGraphLaunch(exec, c->stream);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(c->pinned + k, cnt, sizeof(unsigned), hipMemcpyDeviceToHost,
+                             c->stream);
+        if (e == hipSuccess) e = hipEventRecord(c->gev[k], c->stream);
+        if (e != hipSuccess) {
+          set_err("graph solve: %s", hipGetErrorString(e));
+          (void)hipStreamSynchronize(c->stream);
+          return -(int)e;
+        }
+        ++launched;
+        iter += gi;
+        wave_iter += gi;
       }
-      const hipError_t le = hipGraphLaunch(exec, c->stream);
-      if (le != hipSuccess) { set_err("hipGraphLaunch: %s", hipGetErrorString(le)); return -(int)le; }
-      unsigned total = 0;
-      const hipError_t me = hipMemcpyAsync(&total, cnt, sizeof(total), hipMemcpyDeviceToHost, c->stream);
-      const hipError_t se = me == hipSuccess ? hipStreamSynchronize(c->stream) : me;
+      const uint32_t k = seen & 1u;
+      const hipError_t se = hipEventSynchronize(c->gev[k]);
       if (se != hipSuccess) { set_err("graph solve: %s", hipGetErrorString(se)); return -(int)se; }
-      count = total;
-      iter += gi;
-      wave_iter += gi;
+      count = c->pinned[k];
+      ++seen;
     }
+    const hipError_t se = hipStreamSynchronize(c->stream);
+    if (se != hipSuccess) { set_err("graph solve: %s", hipGetErrorString(se)); return -(int)se; }
     return 0;
   };
 

@@ -1278,12 +1278,13 @@ res_T ssol_mc_shape_get_mc_primitive(struct ssol_mc_shape* sh, const unsigned i,
   return RES_OK;
 }
 
-res_T ssol_solve_sun_batch(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t nsuns,
-                           struct ssol_sun* const* suns, const size_t* realisations,
-                           struct ssol_mc_global* globals, size_t* counts) {
-  if (!scn || !rng || !nsuns || !suns || !realisations || !globals) return RES_BAD_ARG;
+// Checks a sun batch, prepares the scene and flattens the suns; the
+// caller's realisation counts in `nr`
+static res_T batch_setup(struct ssol_scene* scn, const size_t nsuns, struct ssol_sun* const* suns,
+                  const size_t* realisations, const char* who, std::vector<ssolh_sun>& dsuns,
+                  std::vector<uint64_t>& nr) {
   ssol_device* dev = scn->dev;
-  if (res_T rs = check_sun(scn, "ssol_solve_sun_batch")) return rs;
+  if (res_T rs = check_sun(scn, who)) return rs;
   if (nsuns > (1u << 24)) return RES_BAD_ARG;
   uint64_t total = 0;
   for (size_t j = 0; j < nsuns; ++j) {
@@ -1295,26 +1296,46 @@ res_T ssol_solve_sun_batch(struct ssol_scene* scn, const struct ssp_rng* rng, co
   res_T pr = prepare_scene(scn);
   if (pr != RES_OK) return pr;
   Flat& f = dev->cache->flat;
-  std::vector<ssolh_sun> dsuns(nsuns);
+  dsuns.resize(nsuns);
   for (size_t j = 0; j < nsuns; ++j) {
     if (flat_sun(f, suns[j], false, dsuns[j]) != RES_OK) {
-      log_error(dev, "ssol_solve_sun_batch: sun %zu is invalid or its spectrum is not the scene "
-                     "sun's.\n", j);
+      log_error(dev, "%s: sun %zu is invalid or its spectrum is not the scene sun's.\n", who, j);
       return RES_BAD_ARG;
     }
   }
-  std::vector<uint64_t> nr(realisations, realisations + nsuns);
+  nr.assign(realisations, realisations + nsuns);
+  return RES_OK;
+}
+
+static ssolh_batch_args batch_args(const ssol_device* dev, std::vector<ssolh_sun>& dsuns,
+                            std::vector<uint64_t>& nr, uint64_t key, uint64_t counter,
+                            uint32_t slots, uint32_t graph) {
   ssolh_batch_args a{};
-  a.nsuns = (uint32_t)nsuns;
+  a.nsuns = (uint32_t)dsuns.size();
   a.suns = dsuns.data();
   a.realisations = nr.data();
   a.max_failed = 0;
-  a.rng_key = rng->key;
-  a.rng_counter = rng->counter;
+  a.rng_key = key;
+  a.rng_counter = counter;
   a.precision = dev->precision == SSOL_PRECISION_FP32 ? SSOLH_FP32 : SSOLH_FP64;
-  a.path_slots = (uint32_t)std::min<size_t>(dev->path_slots, UINT32_MAX);
+  a.path_slots = slots;
   a.pending_entries = 4;
-  a.graph_iterations = dev->graph_iterations;
+  a.graph_iterations = graph;
+  return a;
+}
+
+res_T ssol_solve_sun_batch(struct ssol_scene* scn, const struct ssp_rng* rng, const size_t nsuns,
+                           struct ssol_sun* const* suns, const size_t* realisations,
+                           struct ssol_mc_global* globals, size_t* counts) {
+  if (!scn || !rng || !nsuns || !suns || !realisations || !globals) return RES_BAD_ARG;
+  ssol_device* dev = scn->dev;
+  std::vector<ssolh_sun> dsuns;
+  std::vector<uint64_t> nr;
+  if (res_T r = batch_setup(scn, nsuns, suns, realisations, "ssol_solve_sun_batch", dsuns, nr))
+    return r;
+  ssolh_batch_args a = batch_args(dev, dsuns, nr, rng->key, rng->counter,
+                                  (uint32_t)std::min<size_t>(dev->path_slots, UINT32_MAX),
+                                  dev->graph_iterations);
   std::vector<ssolh_batch_result> res(nsuns);
   const int e = ssolh_solve_batch(dev->gpu, &a, res.data());
   if (e) {
@@ -1453,6 +1474,55 @@ res_T ssol_device_autotune(struct ssol_device* dev, struct ssol_scene* scn, cons
   out->path_slots = best_slots;
   out->graph_iterations = best_graph;
   out->paths_per_second = (double)probe / best;
+  out->candidates = ntried;
+  return RES_OK;
+}
+
+res_T ssol_device_autotune_batch(struct ssol_device* dev, struct ssol_scene* scn,
+                                 const size_t nsuns, struct ssol_sun* const* suns,
+                                 const size_t* realisations, struct ssol_autotune_result* out) {
+  if (!dev || !scn || scn->dev != dev || !nsuns || !suns || !realisations || !out)
+    return RES_BAD_ARG;
+  std::vector<ssolh_sun> dsuns;
+  std::vector<uint64_t> nr;
+  if (res_T r = batch_setup(scn, nsuns, suns, realisations, "ssol_device_autotune_batch", dsuns,
+                            nr))
+    return r;
+  uint64_t total = 0;
+  for (uint64_t x : nr) total += x;
+  // candidates: wavefront widths (0: the memory-sized default) x iterations
+  // per graph launch (0: host-driven); each timed on the probe batch after
+  // one warm-up, the fastest kept on the device
+  static const uint32_t slot_choices[] = {0, 1u << 22, 1u << 24, 1u << 26};
+  static const uint32_t graph_choices[] = {0, 16, 32};
+  std::vector<ssolh_batch_result> res(nsuns);
+  double best = 1e300;
+  uint32_t best_slots = 0, best_graph = 0;
+  unsigned ntried = 0;
+  for (uint32_t sl : slot_choices) {
+    if (sl && sl >= total) continue;  // the default already holds the whole probe
+    for (uint32_t gi : graph_choices) {
+      ssolh_batch_args a = batch_args(dev, dsuns, nr, 0x6175746f74756e65ull, 0, sl, gi);
+      double t = 0;
+      for (int rep = 0; rep < 2; ++rep) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const int e = ssolh_solve_batch(dev->gpu, &a, res.data());
+        const auto t1 = std::chrono::steady_clock::now();
+        if (e) {
+          fprintf(stderr, "ssol_device_autotune_batch: probe failed: %s\n", ssolh_last_error());
+          return RES_UNKNOWN_ERR;
+        }
+        t = std::chrono::duration<double>(t1 - t0).count();  // keep the second run
+      }
+      ++ntried;
+      if (t < best) { best = t; best_slots = sl; best_graph = gi; }
+    }
+  }
+  dev->path_slots = best_slots;
+  dev->graph_iterations = best_graph;
+  out->path_slots = best_slots;
+  out->graph_iterations = best_graph;
+  out->paths_per_second = (double)total / best;
   out->candidates = ntried;
   return RES_OK;
 }

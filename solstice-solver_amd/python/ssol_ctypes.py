@@ -189,6 +189,8 @@ def _declare(L):
         "ssol_device_get_gpu_context": [VP, C.POINTER(VP)],
         "ssol_device_set_graph_iterations": [VP, C.c_uint],
         "ssol_device_autotune": [VP, VP, C.c_size_t, C.POINTER(AutotuneResult)],
+        "ssol_device_autotune_batch": [VP, VP, C.c_size_t, C.POINTER(VP), C.POINTER(C.c_size_t),
+                                       C.POINTER(AutotuneResult)],
         "ssol_solve_adjoint": [VP, VP, C.c_size_t, C.c_size_t, C.POINTER(VP),
                                C.POINTER(C.c_double), C.c_double, VP, VP,
                                C.POINTER(C.c_double), C.POINTER(C.c_double)],
@@ -406,6 +408,20 @@ class Scene:
             lib().ssol_sun_ref_put(h)
             self._handles.remove(("ssol_sun_ref_put", h))
         return res, list(got)
+
+    def autotune_batch(self, directions, counts) -> AutotuneResult:
+        """ssol_device_autotune_batch on a probe batch of suns like the
+        scene's, pointing along `directions`; the pick stays on the device."""
+        suns = [self._make_sun(self.spec.sun, d) for d in directions]
+        n = len(suns)
+        arr = (VP * n)(*[h.value if isinstance(h, C.c_void_p) else h for h in suns])
+        cnt = (C.c_size_t * n)(*[int(x) for x in counts])
+        out = AutotuneResult()
+        call("ssol_device_autotune_batch", self.dev, self.scn, n, arr, cnt, C.byref(out))
+        for h in suns:
+            lib().ssol_sun_ref_put(h)
+            self._handles.remove(("ssol_sun_ref_put", h))
+        return out
 
     def set_transform(self, inst, transform):
         """ssol_instance_set_transform on the instance built from spec `inst`

@@ -119,3 +119,23 @@ def test_autotune_keeps_a_candidate_and_stays_correct(gpu):
     a = e.global_()["absorbed_by_receivers"]
     assert abs(a[0] - S.CONFIG1_CLOSED_FORM["absorbed_by_receivers"]) <= 4 * a[2] + 1e-6 * 22500
     sc.close()
+
+
+def test_autotune_batch_keeps_a_candidate_and_the_batch_stays_exact(gpu):
+    """ssol_device_autotune_batch times widths x graph lengths on a probe
+    batch and keeps the fastest; the sweep run with the pick equals the sweep
+    run with the defaults (same realisations, FP64 additions reordered)."""
+    spec, dirs, counts = S.config4(n_suns=32, log_min=4.0, log_max=5.0)
+    ref_sc = L.Scene(spec)
+    ref, ref_n = ref_sc.solve_sun_batch(dirs, counts, seed=4)
+    ref_sc.close()
+    sc = L.Scene(spec)
+    t = sc.autotune_batch(dirs[::4], [max(1000, int(c) // 4) for c in counts[::4]])
+    assert t.candidates >= 3 and t.path_slots in (0, 1 << 22, 1 << 24, 1 << 26)
+    assert t.graph_iterations in (0, 16, 32) and t.paths_per_second > 0
+    got, got_n = sc.solve_sun_batch(dirs, counts, seed=4)
+    assert got_n == ref_n
+    for a, b in zip(got, ref):
+        for f in a:
+            assert abs(a[f][0] - b[f][0]) <= 1e-10 * max(1.0, abs(b[f][0])), (f, a[f], b[f])
+    sc.close()
