@@ -8,8 +8,9 @@
  * instances, sun, atmosphere, spectra, ssol_solve and the estimator getters.
  *
  * Differences, all deliberate:
- *  - The rendering API (camera, image, ssol_draw_*) is not provided; it is
- *    not part of flux integration.
+ *  - The rendering API (camera, image, ssol_draw_draft, ssol_draw_pt) runs
+ *    on the solver's GPU kernels; like ssol_solve it needs the scene's sun
+ *    (its spectrum selects the materials' baked values).
  *  - `struct logger` and `struct mem_allocator` stay opaque. They are
  *    accepted by ssol_device_create and ignored: the library allocates with
  *    the C allocator and logs to stderr when `verbose` is set.
@@ -61,6 +62,8 @@ struct mem_allocator;
 struct ssp_rng;
 
 struct ssol_atmosphere;
+struct ssol_camera;
+struct ssol_image;
 struct ssol_device;
 struct ssol_material;
 struct ssol_object;
@@ -702,6 +705,87 @@ SSOL_API res_T ssol_device_autotune_batch
   (struct ssol_device* dev, struct ssol_scene* scn, const size_t nsuns,
    struct ssol_sun* const* suns, const size_t* realisations,
    struct ssol_autotune_result* out);
+
+/* Rendering (reference src/ssol.h: camera 545-575, image 577-631,
+ * ssol_draw_draft / ssol_draw_pt 1225-1244) ---------------------------- */
+enum ssol_pixel_format {
+  SSOL_PIXEL_DOUBLE3,
+  SSOL_PIXEL_FORMATS_COUNT__
+};
+
+enum ssol_filter_mode {
+  SSOL_FILTER_LINEAR,
+  SSOL_FILTER_NEAREST
+};
+
+enum ssol_address_mode {
+  SSOL_ADDRESS_CLAMP,
+  SSOL_ADDRESS_REPEAT
+};
+
+struct ssol_image_layout {
+  size_t row_pitch;     /* bytes between two consecutive rows */
+  size_t offset;        /* byte offset where the image begins */
+  size_t size;          /* width x height, as the reference reports it */
+  size_t width, height; /* pixels in X and Y */
+  enum ssol_pixel_format pixel_format;
+};
+#define SSOL_IMAGE_LAYOUT_NULL__ { 0, 0, 0, 0, 0, SSOL_PIXEL_FORMATS_COUNT__ }
+
+/* Receives `size` pixels (row ordered, X fastest) of the image at `origin` */
+typedef res_T
+(*ssol_write_pixels_T)
+  (void* context, const size_t origin[2], const size_t size[2],
+   const enum ssol_pixel_format fmt, const void* pixels);
+
+static inline size_t
+ssol_sizeof_pixel_format(const enum ssol_pixel_format format)
+{
+  return format == SSOL_PIXEL_DOUBLE3 ? 3 * sizeof(double) : 0;
+}
+
+/* Pinhole camera: at the origin looking along -Z with +Y up and a 90 degree
+ * horizontal field of view until set */
+SSOL_API res_T ssol_camera_create(struct ssol_device* dev, struct ssol_camera** cam);
+SSOL_API res_T ssol_camera_ref_get(struct ssol_camera* cam);
+SSOL_API res_T ssol_camera_ref_put(struct ssol_camera* cam);
+SSOL_API res_T ssol_camera_set_proj_ratio(struct ssol_camera* cam, const double proj_ratio);
+SSOL_API res_T ssol_camera_set_fov(struct ssol_camera* cam, const double fov); /* radians */
+SSOL_API res_T ssol_camera_look_at
+  (struct ssol_camera* cam, const double position[3], const double target[3],
+   const double up[3]);
+
+SSOL_API res_T ssol_image_create(struct ssol_device* dev, struct ssol_image** image);
+SSOL_API res_T ssol_image_ref_get(struct ssol_image* image);
+SSOL_API res_T ssol_image_ref_put(struct ssol_image* image);
+SSOL_API res_T ssol_image_setup
+  (struct ssol_image* image, const size_t width, const size_t height,
+   const enum ssol_pixel_format format);
+SSOL_API res_T ssol_image_get_layout
+  (const struct ssol_image* image, struct ssol_image_layout* layout);
+SSOL_API res_T ssol_image_map(const struct ssol_image* image, char** memory);
+SSOL_API res_T ssol_image_unmap(const struct ssol_image* image);
+SSOL_API res_T ssol_image_sample
+  (const struct ssol_image* image, const enum ssol_filter_mode filter,
+   const enum ssol_address_mode address_u, const enum ssol_address_mode address_v,
+   const double uv[2], void* val);
+/* Matches ssol_write_pixels_T: copies the pixels into the image */
+SSOL_API res_T ssol_image_write
+  (void* image, const size_t origin[2], const size_t size[2],
+   const enum ssol_pixel_format fmt, const void* pixels);
+
+/* Per pixel, the mean over `spp` samples of the cosine between the first
+ * hit's normal and the view ray (0 on a miss) */
+SSOL_API res_T ssol_draw_draft
+  (struct ssol_scene* scn, struct ssol_camera* cam, const size_t width,
+   const size_t height, const size_t spp, ssol_write_pixels_T writer, void* writer_data);
+
+/* Path-traced radiance per pixel: unit sun sampled at every hit, sky 0.1
+ * towards `up` */
+SSOL_API res_T ssol_draw_pt
+  (struct ssol_scene* scn, struct ssol_camera* cam, const size_t width,
+   const size_t height, const size_t spp, const double up[3],
+   ssol_write_pixels_T writer, void* writer_data);
 
 #ifdef __cplusplus
 }

@@ -356,6 +356,27 @@ SSOLH_API int ssolh_scene_move_instances
  * ssolh_scene_move_instances left on the device; returns after the build. */
 SSOLH_API int ssolh_scene_build_tlas(ssolh_context* ctx);
 
+/* Image of the uploaded scene through a pinhole camera (reference
+ * ssol_draw_draft, src/ssol_draw_draft.c, and ssol_draw_pt,
+ * src/ssol_draw_pt.c:126-285), traced by the solver's kernels. Pixel
+ * (x, y) averages `spp` camera rays through (x + jx, y + jy) / (width,
+ * height) of the image plane, ray direction = axis_x (2 sx - 1) + axis_y
+ * (2 sy - 1) + axis_z (the reference camera_ray). mode 0 (draft): the
+ * cosine between the first hit's normal and the view ray, offsets
+ * draft_samples[2k], [2k + 1] shared by every pixel; mode 1 (path
+ * tracing): offsets and paths from Threefry streams keyed by (rng_key,
+ * pixel * spp + k), sky 0.1 towards `up`, the unit sun sampled at every
+ * hit. pixels: [height][width] host doubles, one radiance per pixel. */
+struct ssolh_render_args {
+  uint32_t width, height, spp, mode;
+  float position[3], axis_x[3], axis_y[3], axis_z[3];
+  double up[3];
+  uint64_t rng_key;
+  const float* draft_samples;   /* [spp][2], mode 0 */
+};
+SSOLH_API int ssolh_render(ssolh_context* ctx, const struct ssolh_render_args* args,
+                           double* pixels);
+
 /* Run the realisations of one solve over the uploaded scene */
 SSOLH_API int ssolh_solve(ssolh_context* ctx, const struct ssolh_solve_args* args);
 

@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, "lib")
 
 HIP_SOURCES = [os.path.join(HERE, "csrc", "hip", "ssolh.hip")]
 HOST_SOURCES = [os.path.join(HERE, "csrc", "host", f) for f in (
-    "ssol_objects.cpp", "ssol_shape.cpp", "ssol_bvh.cpp", "ssol_solve.cpp", "ssp_rng.cpp")]
+    "ssol_objects.cpp", "ssol_shape.cpp", "ssol_bvh.cpp", "ssol_solve.cpp", "ssol_draw.cpp",
+    "ssp_rng.cpp")]
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
                "-munsafe-fp-atomics", "-fvisibility=hidden",

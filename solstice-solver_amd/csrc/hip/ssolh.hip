@@ -1983,6 +1983,7 @@ k_scatter(const uint32_t* __restrict__ list, const unsigned* __restrict__ count_
 }
 
 #include "ssolh_fp32.hpp"
+#include "ssolh_render.hpp"
 
 // First work list: slots [0, n) in order, and its length
 __global__ void k_iota(uint32_t* __restrict__ list, uint32_t n, unsigned* __restrict__ count) {
@@ -2229,6 +2230,7 @@ struct ssolh_context {
   // the BLASes' part of the stack bound, the build's work buffer
   uint32_t tlas_base = 0, tlas_cap = 0, blas_pb = 0;
   DevBuf lbvh;
+  DevBuf render;  // renderer work buffer (ssolh_render)
   // solve buffers
   DevBuf start_hist;  // per-instance histogram / offsets of the first realisations
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, scan_chunks, counters,
@@ -2558,7 +2560,7 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
   free_scene(c);
   for (DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
                     &c->block_emits, &c->block_offsets, &c->scan_chunks, &c->counters, &c->tallies,
-                    &c->jobs, &c->start_hist, &c->trace_ovf, &c->lbvh,
+                    &c->jobs, &c->start_hist, &c->trace_ovf, &c->lbvh, &c->render,
                     &c->adj_state, &c->adj_in, &c->adj_grad, &c->trk_vtx, &c->trk_type,
                     &c->trk_seq, &c->trk_count, &c->trk_pos})
     if (b->p) (void)hipFree(b->p);
@@ -2823,6 +2825,90 @@ SSOLH_API int ssolh_scene_build_tlas(ssolh_context* c) {
   // a descent pushes at most 3 per 4-wide level above the leaves
   c->stack_depth = 3u * (h.depth4 + 1u) + 1u + c->blas_pb;
   ++c->n_builds;
+  return 0;
+}
+
+SSOLH_API int ssolh_render(ssolh_context* c, const struct ssolh_render_args* a, double* pixels) {
+  if (!c || !a || !pixels || !a->width || !a->height || !a->spp || a->mode > 1 ||
+      (a->mode == 0 && !a->draft_samples)) {
+    set_err("ssolh_render: invalid arguments");
+    return SSOLH_ERR_ARG;
+  }
+  if (!c->has_scene) { set_err("ssolh_render: no scene"); return SSOLH_ERR_ARG; }
+  const uint64_t npix = (uint64_t)a->width * a->height, total = npix * a->spp;
+  if (npix > (1u << 28) || a->spp > (1u << 20)) {
+    set_err("ssolh_render: %llu pixels x %u samples", (unsigned long long)npix, a->spp);
+    return SSOLH_ERR_LIMIT;
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  const uint32_t batch = (uint32_t)std::min<uint64_t>(total, 1u << 21);
+  const uint32_t nslot = 2 * batch;
+  const uint32_t lds_stack = trace_lds_stack(c);
+  const uint32_t extra = c->stack_depth > lds_stack ? c->stack_depth - lds_stack : 0;
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t sizes[] = {(size_t)batch * sizeof(RenderState), (size_t)nslot * sizeof(RayRec),
+                          (size_t)nslot * sizeof(HitRec), (size_t)nslot * 4, (size_t)nslot * 4,
+                          npix * 8, (size_t)a->spp * 8, 16, (size_t)nslot * extra * 4 + 4};
+  size_t bytes = 0;
+  for (size_t b : sizes) bytes += al(b);
+  if (int e = dev_alloc(c->render, bytes)) return e;
+  char* p = (char*)c->render.p;
+  void* part[9];
+  for (int i = 0; i < 9; ++i) { part[i] = p; p += al(sizes[i]); }
+  RenderState* st = (RenderState*)part[0];
+  RayRec* rays = (RayRec*)part[1];
+  HitRec* hits = (HitRec*)part[2];
+  uint32_t* lists[2] = {(uint32_t*)part[3], (uint32_t*)part[4]};
+  double* pix = (double*)part[5];
+  float* draft = (float*)part[6];
+  unsigned* cnt = (unsigned*)part[7];  // [0], [1]: list lengths, [2]: unused ticket
+  const TOvf ovf{extra ? (uint32_t*)part[8] : nullptr, nslot, extra};
+  HIP_TRY(hipMemsetAsync(pix, 0, npix * 8, c->stream));
+  if (a->mode == 0)
+    HIP_TRY(hipMemcpyAsync(draft, a->draft_samples, (size_t)a->spp * 8, hipMemcpyHostToDevice,
+                           c->stream));
+  RenderParams rp{};
+  rp.width = a->width; rp.height = a->height; rp.spp = a->spp; rp.mode = a->mode;
+  for (int k = 0; k < 3; ++k) {
+    rp.pos[k] = a->position[k]; rp.ax[k] = a->axis_x[k]; rp.ay[k] = a->axis_y[k];
+    rp.az[k] = a->axis_z[k]; rp.up[k] = a->up[k];
+  }
+  rp.key = a->rng_key;
+  rp.draft = draft;
+  for (uint64_t first = 0; first < total; first += batch) {
+    rp.first = first;
+    rp.count = (uint32_t)std::min<uint64_t>(total - first, batch);
+    const uint32_t blocks = (rp.count + BLOCK - 1) / BLOCK;
+    HIP_TRY(hipMemsetAsync(cnt, 0, 16, c->stream));
+    k_render_start<<<blocks, BLOCK, 0, c->stream>>>(c->scn, rp, st, rays, lists[0], cnt);
+    HIP_TRY(hipGetLastError());
+    int cur = 0;
+    for (uint32_t iter = 0;; ++iter) {
+      unsigned n = 0;
+      HIP_TRY(hipMemcpyAsync(&n, cnt + cur, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      if (!n) break;
+      if (iter >= 100000) { set_err("ssolh_render: paths did not end"); return SSOLH_ERR_LIMIT; }
+      const uint32_t tb = (n + BLOCK - 1) / BLOCK;
+      if (lds_stack == 16)
+        k_trace<16, false><<<tb, BLOCK, 0, c->stream>>>(c->scn, lists[cur], cnt + cur, rays, hits,
+                                                       cnt + 2, ovf);
+      else if (lds_stack == 24)
+        k_trace<24, false><<<tb, BLOCK, 0, c->stream>>>(c->scn, lists[cur], cnt + cur, rays, hits,
+                                                       cnt + 2, ovf);
+      else
+        k_trace<MAX_STACK, false><<<tb, BLOCK, 0, c->stream>>>(c->scn, lists[cur], cnt + cur,
+                                                              rays, hits, cnt + 2, ovf);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemsetAsync(cnt + (cur ^ 1), 0, sizeof(unsigned), c->stream));
+      k_render_shade<<<blocks, BLOCK, 0, c->stream>>>(c->scn, rp, st, rays, hits, pix,
+                                                      lists[cur ^ 1], cnt + (cur ^ 1));
+      HIP_TRY(hipGetLastError());
+      cur ^= 1;
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(pixels, pix, npix * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }
 

@@ -184,6 +184,10 @@ template <typename T> inline void ref_get(T* o) { ++o->ref; }
 
 void device_ref_put_(ssol_device* d);
 void solve_cache_free(ssol_device* d);
+// The scene flattened and on the GPU as ssol_solve would have it (sun
+// checked, cache reused or refreshed, context created); `who` names the
+// caller in log lines (ssol_solve.cpp)
+res_T scene_for_gpu(ssol_scene* scn, const char* who);
 uint64_t next_gen();  // monotonic stamp for ssol_refcounted::gen
 
 // Punched-surface tessellation (ssol_shape.cpp)

@@ -1478,6 +1478,18 @@ res_T ssol_device_autotune(struct ssol_device* dev, struct ssol_scene* scn, cons
   return RES_OK;
 }
 
+}  // extern "C"
+
+namespace ssol {
+res_T scene_for_gpu(ssol_scene* scn, const char* who) {
+  if (!scn) return RES_BAD_ARG;
+  if (res_T rs = check_sun(scn, who)) return rs;
+  return prepare_scene(scn);
+}
+}  // namespace ssol
+
+extern "C" {
+
 res_T ssol_device_autotune_batch(struct ssol_device* dev, struct ssol_scene* scn,
                                  const size_t nsuns, struct ssol_sun* const* suns,
                                  const size_t* realisations, struct ssol_autotune_result* out) {

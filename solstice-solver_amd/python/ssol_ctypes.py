@@ -183,6 +183,24 @@ PVP = C.POINTER(C.c_void_p)
 def _declare(L):
     sigs = {
         "ssol_device_create": [VP, VP, C.c_uint, C.c_int, PVP],
+        "ssol_camera_create": [VP, PVP],
+        "ssol_camera_ref_put": [VP],
+        "ssol_camera_set_proj_ratio": [VP, C.c_double],
+        "ssol_camera_set_fov": [VP, C.c_double],
+        "ssol_camera_look_at": [VP, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                C.POINTER(C.c_double)],
+        "ssol_image_create": [VP, PVP],
+        "ssol_image_ref_put": [VP],
+        "ssol_image_setup": [VP, C.c_size_t, C.c_size_t, C.c_int],
+        "ssol_image_get_layout": [VP, VP],
+        "ssol_image_map": [VP, C.POINTER(VP)],
+        "ssol_image_unmap": [VP],
+        "ssol_image_sample": [VP, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                              C.POINTER(C.c_double)],
+        "ssol_image_write": [VP, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.c_int, VP],
+        "ssol_draw_draft": [VP, VP, C.c_size_t, C.c_size_t, C.c_size_t, VP, VP],
+        "ssol_draw_pt": [VP, VP, C.c_size_t, C.c_size_t, C.c_size_t, C.POINTER(C.c_double),
+                         VP, VP],
         "ssol_device_ref_put": [VP],
         "ssol_device_set_precision": [VP, C.c_int],
         "ssol_device_set_wavefront_size": [VP, C.c_size_t],
@@ -422,6 +440,34 @@ class Scene:
             lib().ssol_sun_ref_put(h)
             self._handles.remove(("ssol_sun_ref_put", h))
         return out
+
+    def draw(self, width, height, spp, position, target, up, fov=None, proj_ratio=None,
+             sky_up=None):
+        """ssol_draw_draft (sky_up None) or ssol_draw_pt into an ssol_image
+        through ssol_image_write; returns the image as [height][width][3]"""
+        cam = new("ssol_camera_create", self.dev)
+        img = new("ssol_image_create", self.dev)
+        try:
+            d3 = lambda v: (C.c_double * 3)(*[float(x) for x in v])  # noqa: E731
+            if fov is not None:
+                call("ssol_camera_set_fov", cam, fov)
+            if proj_ratio is not None:
+                call("ssol_camera_set_proj_ratio", cam, proj_ratio)
+            call("ssol_camera_look_at", cam, d3(position), d3(target), d3(up))
+            call("ssol_image_setup", img, width, height, 0)
+            writer = C.cast(lib().ssol_image_write, VP)
+            if sky_up is None:
+                call("ssol_draw_draft", self.scn, cam, width, height, spp, writer, img)
+            else:
+                call("ssol_draw_pt", self.scn, cam, width, height, spp, d3(sky_up), writer, img)
+            mem = C.c_void_p()
+            call("ssol_image_map", img, C.byref(mem))
+            buf = C.string_at(mem.value, width * height * 24)
+            call("ssol_image_unmap", img)
+            return np.frombuffer(buf, dtype=np.float64).reshape(height, width, 3).copy()
+        finally:
+            lib().ssol_image_ref_put(img)
+            lib().ssol_camera_ref_put(cam)
 
     def set_transform(self, inst, transform):
         """ssol_instance_set_transform on the instance built from spec `inst`
