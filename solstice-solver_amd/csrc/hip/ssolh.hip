@@ -1458,6 +1458,11 @@ __device__ __forceinline__ void shade_block(
   fin.job = 0;
   for (int i = 0; i < 6; ++i) fin.g[i] = 0;
   const uint32_t slot = live ? slot_of(list, q) : 0;
+  // receiver weights of a hit that ends the path: flushed from registers
+  // below rather than stored as pending entries and read back (dkey, dpkey:
+  // receiver and per-primitive keys, NONE: no such hit)
+  uint32_t dkey = NONE, dpkey = NONE;
+  double din[4] = {0, 0, 0, 0};  // in0, in1, in2, kabs of that hit
 
   uint64_t path = 0;
   if (live) {
@@ -1651,7 +1656,13 @@ __device__ __forceinline__ void shade_block(
           w10[5] = in0 * kabs; w10[6] = in1 * kabs; w10[7] = in2 * kabs;
           w10[8] = (in2 - in0) * kabs; w10[9] = (in1 - in0) * kabs;
           const uint32_t rkey = (uint32_t)is.receiver * 2u + (side == SSOL_FRONT_ ? 0u : 1u);
-          if (!pend_add(P.pkey + slot, P.pval + slot, stride, prm.npend, rkey, w10)) {
+          // the path ends at this point (weight zero below): its weights
+          // join the end-of-path flush directly
+          const bool direct = out0 == 0;
+          if (direct) {
+            dkey = rkey;
+            din[0] = in0; din[1] = in1; din[2] = in2; din[3] = kabs;
+          } else if (!pend_add(P.pkey + slot, P.pval + slot, stride, prm.npend, rkey, w10)) {
             load_aux();
             spill(T.rcv + (size_t)(blockIdx.x & T.mask_r) * T.nrcv * 40u, rkey, w10, &C->spills);
             spill(T.sxr + (size_t)(blockIdx.x & T.mask_x) * T.nsamp * T.nrcv * 40u, (uint64_t)aux.x * S.nrcv * 2u + rkey, w10, &C->spills);
@@ -1659,7 +1670,9 @@ __device__ __forceinline__ void shade_block(
           if (is.prim_tally >= 0) {
             const uint32_t pkey = (uint32_t)(((uint64_t)is.prim_tally + prim) * 2u +
                                              (side == SSOL_FRONT_ ? 0u : 1u));
-            if (!pend_add(P.qkey + slot, P.qval + slot, stride, prm.npend, pkey, w10))
+            if (direct)
+              dpkey = pkey;
+            else if (!pend_add(P.qkey + slot, P.qval + slot, stride, prm.npend, pkey, w10))
               spill(T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u, pkey, w10, &C->spills);
             if (A.grad && A.p[slot] != NONE) A.s[slot].w += A.gmap[pkey] * w10[5];
           }
@@ -1755,52 +1768,81 @@ __device__ __forceinline__ void shade_block(
   // pending entries fill from entry 0 up (pend_add takes the lowest free
   // one), so a path's keys end at its first empty entry and the wave stops
   // at the first entry none of its finished paths uses
+  // The weights of the hit that ended the path (a direct entry) join the
+  // pending entry of the same key if the path holds one, before squaring;
+  // else they take the path's first empty entry position, one past the
+  // pending entries at most.
   bool more = live && fin.kind != FIN_NONE;
-  for (uint32_t k = 0; k < prm.npend; ++k) {
+  bool dpend = done && dkey != NONE;
+  double dw[10];
+  dw[0] = din[0]; dw[1] = din[1]; dw[2] = din[2];
+  dw[3] = din[2] - din[0]; dw[4] = din[1] - din[0];
+  dw[5] = din[0] * din[3]; dw[6] = din[1] * din[3]; dw[7] = din[2] * din[3];
+  dw[8] = (din[2] - din[0]) * din[3]; dw[9] = (din[1] - din[0]) * din[3];
+  for (uint32_t k = 0; k <= prm.npend; ++k) {
     uint32_t key = NONE;
-    double v[20];
-    if (more) {
+    double a10[10];
+    if (more && k < prm.npend) {
       key = P.pkey[k * stride + slot];
       if (key != NONE) {
-        for (int f = 0; f < 10; ++f) {
-          const double x = P.pval[((size_t)k * 10 + f) * stride + slot] * fin.factor;
-          v[2 * f] = x;
-          v[2 * f + 1] = x * x;
-        }
+        const bool merge = dpend && key == dkey;
+        for (int f = 0; f < 10; ++f)
+          a10[f] = P.pval[((size_t)k * 10 + f) * stride + slot] + (merge ? dw[f] : 0.0);
+        if (merge) dpend = false;
         P.pkey[k * stride + slot] = NONE;
-      } else {
-        more = false;
+      }
+    }
+    if (key == NONE) {
+      more = false;
+      if (dpend) {
+        key = dkey;
+        for (int f = 0; f < 10; ++f) a10[f] = dw[f];
+        dpend = false;
       }
     }
     if (!__ballot(key != NONE)) break;
     const bool has = done && key != NONE;
-    if (!has) for (int f = 0; f < 20; ++f) v[f] = 0;
+    double v[20];
+    for (int f = 0; f < 10; ++f) {
+      const double x = has ? a10[f] * fin.factor : 0.0;
+      v[2 * f] = x;
+      v[2 * f + 1] = x * x;
+    }
     keyed_add<20>(has, has ? key : 0, v, rcv_base);
     const uint64_t xkey = has ? (uint64_t)fin.samp_slot * (uint64_t)S.nrcv * 2u + key : 0;
     keyed_add<20>(has, xkey, v, T.sxr + (size_t)(blockIdx.x & T.mask_x) * T.nsamp * T.nrcv * 40u);
   }
   more = live && fin.kind != FIN_NONE;  // per-primitive entries, the same way
-  for (uint32_t k = 0; k < prm.npend; ++k) {
+  dpend = done && dpkey != NONE;
+  for (uint32_t k = 0; k <= prm.npend; ++k) {
     uint32_t key = NONE;
-    double v[20];
-    if (more) {
+    double a10[10];
+    if (more && k < prm.npend) {
       key = P.qkey[k * stride + slot];
       if (key != NONE) {
-        for (int f = 0; f < 10; ++f) {
-          const double x = P.qval[((size_t)k * 10 + f) * stride + slot] * fin.factor;
-          v[2 * f] = x;
-          v[2 * f + 1] = x * x;
-        }
+        const bool merge = dpend && key == dpkey;
+        for (int f = 0; f < 10; ++f)
+          a10[f] = P.qval[((size_t)k * 10 + f) * stride + slot] + (merge ? dw[f] : 0.0);
+        if (merge) dpend = false;
         P.qkey[k * stride + slot] = NONE;
-      } else {
-        more = false;
+      }
+    }
+    if (key == NONE) {
+      more = false;
+      if (dpend) {
+        key = dpkey;
+        for (int f = 0; f < 10; ++f) a10[f] = dw[f];
+        dpend = false;
       }
     }
     if (!__ballot(key != NONE)) break;
-    const bool has = done && key != NONE;
-    if (has) {
+    if (done && key != NONE) {
       double* pb = T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u;
-      for (int f = 0; f < 20; ++f) atomic_add(pb + (size_t)key * 20 + f, v[f]);
+      for (int f = 0; f < 10; ++f) {
+        const double x = a10[f] * fin.factor;
+        atomic_add(pb + (size_t)key * 20 + 2 * f, x);
+        atomic_add(pb + (size_t)key * 20 + 2 * f + 1, x * x);
+      }
     }
   }
 
