@@ -474,6 +474,10 @@ __device__ __forceinline__ void shade32_block(
   fin.job = 0;
   for (int i = 0; i < 6; ++i) fin.g[i] = 0;
   const uint32_t slot = live ? slot_of(list, q) : 0;
+  // receiver weights of a hit that ends the path, flushed from registers (as
+  // shade_block)
+  uint32_t dkey = NONE, dpkey = NONE;
+  float din[4] = {0, 0, 0, 0};  // in0, in1, in2, kabs of that hit
 
   if (live) {
     const RayRec32 r = rays[slot];
@@ -615,14 +619,20 @@ __device__ __forceinline__ void shade32_block(
           w10[5] = in0 * kabs; w10[6] = in1 * kabs; w10[7] = in2 * kabs;
           w10[8] = (in2 - in0) * kabs; w10[9] = (in1 - in0) * kabs;
           const uint32_t rkey = (uint32_t)is.receiver * 2u + (side == SSOL_FRONT_ ? 0u : 1u);
-          if (!pend_add32(P.pkey + slot, P.pval + slot, stride, prm.npend, rkey, w10)) {
+          const bool direct = out0 == 0;  // the path ends here
+          if (direct) {
+            dkey = rkey;
+            din[0] = in0; din[1] = in1; din[2] = in2; din[3] = kabs;
+          } else if (!pend_add32(P.pkey + slot, P.pval + slot, stride, prm.npend, rkey, w10)) {
             spill32(T.rcv + (size_t)(blockIdx.x & T.mask_r) * T.nrcv * 40u, rkey, w10, &C->spills);
             spill32(T.sxr + (size_t)(blockIdx.x & T.mask_x) * T.nsamp * T.nrcv * 40u, (uint64_t)cur.w * S.nrcv * 2u + rkey, w10, &C->spills);
           }
           if (is.prim_tally >= 0) {
             const uint32_t pkey = (uint32_t)(((uint64_t)is.prim_tally + prim) * 2u +
                                              (side == SSOL_FRONT_ ? 0u : 1u));
-            if (!pend_add32(P.qkey + slot, P.qval + slot, stride, prm.npend, pkey, w10))
+            if (direct)
+              dpkey = pkey;
+            else if (!pend_add32(P.qkey + slot, P.qval + slot, stride, prm.npend, pkey, w10))
               spill32(T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u, pkey, w10, &C->spills);
           }
         } else {
@@ -703,53 +713,84 @@ __device__ __forceinline__ void shade32_block(
     const unsigned long long nf = wave_count_add(fin.kind == FIN_FAILED, &C->failed);
     if (fin.kind == FIN_FAILED && nf + 1 >= prm.max_failed) atomicExch(&C->aborted, 1u);
   }
-  // pending entries end at a path's first empty one (as shade_block)
+  // pending entries end at a path's first empty one; the direct entry joins
+  // the pending entry of its key (summed in FP32 as pend_add32 would) or
+  // takes the first empty position (as shade_block)
   bool more = live && fin.kind != FIN_NONE;
-  for (uint32_t k = 0; k < prm.npend; ++k) {
+  bool dpend = done && dkey != NONE;
+  float dw[10];
+  dw[0] = din[0]; dw[1] = din[1]; dw[2] = din[2];
+  dw[3] = din[2] - din[0]; dw[4] = din[1] - din[0];
+  dw[5] = din[0] * din[3]; dw[6] = din[1] * din[3]; dw[7] = din[2] * din[3];
+  dw[8] = (din[2] - din[0]) * din[3]; dw[9] = (din[1] - din[0]) * din[3];
+  for (uint32_t k = 0; k <= prm.npend; ++k) {
     uint32_t key = NONE;
-    double v[20];
-    if (more) {
+    float a10[10];
+    if (more && k < prm.npend) {
       key = P.pkey[k * stride + slot];
       if (key != NONE) {
+        const bool merge = dpend && key == dkey;
         for (int f = 0; f < 10; ++f) {
-          const double x = (double)P.pval[((size_t)k * 10 + f) * stride + slot] * fin.factor;
-          v[2 * f] = x;
-          v[2 * f + 1] = x * x;
+          const float x = P.pval[((size_t)k * 10 + f) * stride + slot];
+          a10[f] = merge ? x + dw[f] : x;
         }
+        if (merge) dpend = false;
         P.pkey[k * stride + slot] = NONE;
-      } else {
-        more = false;
+      }
+    }
+    if (key == NONE) {
+      more = false;
+      if (dpend) {
+        key = dkey;
+        for (int f = 0; f < 10; ++f) a10[f] = dw[f];
+        dpend = false;
       }
     }
     if (!__ballot(key != NONE)) break;
     const bool has = done && key != NONE;
-    if (!has) for (int f = 0; f < 20; ++f) v[f] = 0;
+    double v[20];
+    for (int f = 0; f < 10; ++f) {
+      const double x = has ? (double)a10[f] * fin.factor : 0.0;
+      v[2 * f] = x;
+      v[2 * f + 1] = x * x;
+    }
     keyed_add<20>(has, has ? key : 0, v, rcv_base);
     const uint64_t xkey = has ? (uint64_t)fin.samp_slot * (uint64_t)S.nrcv * 2u + key : 0;
     keyed_add<20>(has, xkey, v, T.sxr + (size_t)(blockIdx.x & T.mask_x) * T.nsamp * T.nrcv * 40u);
   }
   more = live && fin.kind != FIN_NONE;
-  for (uint32_t k = 0; k < prm.npend; ++k) {
+  dpend = done && dpkey != NONE;
+  for (uint32_t k = 0; k <= prm.npend; ++k) {
     uint32_t key = NONE;
-    double v[20];
-    if (more) {
+    float a10[10];
+    if (more && k < prm.npend) {
       key = P.qkey[k * stride + slot];
       if (key != NONE) {
+        const bool merge = dpend && key == dpkey;
         for (int f = 0; f < 10; ++f) {
-          const double x = (double)P.qval[((size_t)k * 10 + f) * stride + slot] * fin.factor;
-          v[2 * f] = x;
-          v[2 * f + 1] = x * x;
+          const float x = P.qval[((size_t)k * 10 + f) * stride + slot];
+          a10[f] = merge ? x + dw[f] : x;
         }
+        if (merge) dpend = false;
         P.qkey[k * stride + slot] = NONE;
-      } else {
-        more = false;
+      }
+    }
+    if (key == NONE) {
+      more = false;
+      if (dpend) {
+        key = dpkey;
+        for (int f = 0; f < 10; ++f) a10[f] = dw[f];
+        dpend = false;
       }
     }
     if (!__ballot(key != NONE)) break;
-    const bool has = done && key != NONE;
-    if (has) {
+    if (done && key != NONE) {
       double* pb = T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u;
-      for (int f = 0; f < 20; ++f) atomic_add(pb + (size_t)key * 20 + f, v[f]);
+      for (int f = 0; f < 10; ++f) {
+        const double x = (double)a10[f] * fin.factor;
+        atomic_add(pb + (size_t)key * 20 + 2 * f, x);
+        atomic_add(pb + (size_t)key * 20 + 2 * f + 1, x * x);
+      }
     }
   }
 
