@@ -222,6 +222,14 @@ struct Tallies {
   uint32_t mask_g, mask_r, mask_s, mask_x, mask_p;
   uint32_t nrcv, nsamp, njobs;
   uint64_t nprim;
+  // Log of finished paths' per-primitive entries (prim_add), reduced into
+  // `prim` by key after sorting (k_prim_reduce): record i is the key
+  // log_key[i] and the 10 fields' weights log_val[10 i ..]. log_n counts the
+  // records asked for, also past log_cap (those went to atomics).
+  uint32_t* log_key;
+  double* log_val;
+  unsigned long long* log_n;
+  uint64_t log_cap;
 };
 
 struct SolveParams {
@@ -373,6 +381,81 @@ __device__ __forceinline__ unsigned long long wave_count_add(bool pred, unsigned
   base = __shfl(base, leader, 64);
   const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
   return base + (unsigned long long)__popcll(below);
+}
+
+// A finished path's per-primitive entry: the key and the 10 fields' weights
+// a[f] * factor. FP64 atomics issue at one device-wide rate whatever their
+// addresses, and the entries of a wave fall on as many primitives as it has
+// lanes, so the 20 atomics of an entry (x and x^2 per field) would cost far
+// more than the 84 bytes of a record in the log, which k_prim_reduce sums
+// by key after sorting. A full log takes the atomics. Must be called by the
+// whole wave.
+template <typename R>
+__device__ void prim_add(const Tallies& T, bool has, uint32_t key, const R* a, double factor) {
+  const unsigned long long pos = T.log_cap ? wave_count_add(has, T.log_n) : ~0ull;
+  if (!has) return;
+  if (pos < T.log_cap) {
+    T.log_key[pos] = key;
+    double2* v = (double2*)(T.log_val + pos * 10u);
+#pragma unroll
+    for (int f = 0; f < 5; ++f)
+      v[f] = make_double2((double)a[2 * f] * factor, (double)a[2 * f + 1] * factor);
+    return;
+  }
+  double* pb = T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u + (size_t)key * 20u;
+#pragma unroll
+  for (int f = 0; f < 10; ++f) {
+    const double x = (double)a[f] * factor;
+    atomic_add(pb + 2 * f, x);
+    atomic_add(pb + 2 * f + 1, x * x);
+  }
+}
+
+// The sorted log (keys skey, record indices sidx, m records): each wave sums
+// the runs of equal keys among its 64 records (x and x^2 of the 10 fields)
+// and adds each run with one set of 20 atomics.
+__global__ void __launch_bounds__(256) k_prim_reduce(const uint32_t* __restrict__ skey,
+                                                     const uint32_t* __restrict__ sidx,
+                                                     const double* __restrict__ val, uint32_t m,
+                                                     double* prim, uint32_t mask_p, uint64_t nprim) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < m;
+  const uint32_t key = in ? skey[i] : 0u;
+  double x[10];
+  if (in) {
+    const double2* v = (const double2*)(val + (size_t)sidx[i] * 10u);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      const double2 t = v[f];
+      x[2 * f] = t.x;
+      x[2 * f + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < 10; ++f) x[f] = 0.0;
+  }
+  double* pb = prim + (size_t)(blockIdx.x & mask_p) * nprim * 40u;
+  bool todo = in;
+  for (uint64_t mask = __ballot(todo); mask; mask = __ballot(todo)) {
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    const uint32_t lkey = __shfl(key, leader, 64);
+    const bool match = todo && key == lkey;
+#pragma unroll
+    for (int f = 0; f < 10; ++f) {
+      const double s1 = wave_sum(match ? x[f] : 0.0);
+      const double s2 = wave_sum(match ? x[f] * x[f] : 0.0);
+      if ((int)lane_id() == leader) {
+        atomic_add(pb + (size_t)lkey * 20u + 2 * f, s1);
+        atomic_add(pb + (size_t)lkey * 20u + 2 * f + 1, s2);
+      }
+    }
+    todo = todo && !match;
+  }
+}
+
+__global__ void k_iota32(uint32_t* a, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
 }
 
 // The logical blocks (of BLOCK list entries) of a looping, graph-captured
@@ -1836,14 +1919,7 @@ __device__ __forceinline__ void shade_block(
       }
     }
     if (!__ballot(key != NONE)) break;
-    if (done && key != NONE) {
-      double* pb = T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u;
-      for (int f = 0; f < 10; ++f) {
-        const double x = a10[f] * fin.factor;
-        atomic_add(pb + (size_t)key * 20 + 2 * f, x);
-        atomic_add(pb + (size_t)key * 20 + 2 * f + 1, x * x);
-      }
-    }
+    prim_add(T, done && key != NONE, key, a10, fin.factor);
   }
 
   // --- adjoint: the finished path's weighted contribution times its scores,
@@ -2232,6 +2308,8 @@ struct DevBuf {
 // copy is reached through the pointer hipGetSymbolAddress returns with that
 // device current (kept per device by ssolh_create).
 constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+// records of the per-primitive log (84 B each plus 12 B of sort arrays)
+constexpr uint64_t LOG_MAX_RECORDS = uint64_t(1) << 25;
 static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
 
 constexpr int MAX_DEVICES = 64;
@@ -2277,6 +2355,14 @@ struct ssolh_context {
   DevBuf start_hist;  // per-instance histogram / offsets of the first realisations
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, scan_chunks, counters,
       tallies, jobs, trace_ovf;
+  // per-primitive log: keys, sorted keys, indices, sorted indices, values,
+  // count, sort work (Tallies::log_*); log_sort, log_sidx, log_skey and
+  // log_idx within it, log_sort_bytes of sort work
+  DevBuf plog;
+  void* log_sort = nullptr;
+  size_t log_sort_bytes = 0;
+  uint32_t *log_skey = nullptr, *log_idx = nullptr, *log_sidx = nullptr;
+  uint32_t log_key_bits = 1;
   DevBuf adj_state, adj_in, adj_grad;
   DevBuf trk_vtx, trk_type, trk_seq, trk_count, trk_pos;
   uint64_t trk_paths = 0;           // realisations of the last tracked solve
@@ -2602,7 +2688,7 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
   free_scene(c);
   for (DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
                     &c->block_emits, &c->block_offsets, &c->scan_chunks, &c->counters, &c->tallies,
-                    &c->jobs, &c->start_hist, &c->trace_ovf, &c->lbvh, &c->render,
+                    &c->jobs, &c->start_hist, &c->trace_ovf, &c->lbvh, &c->render, &c->plog,
                     &c->adj_state, &c->adj_in, &c->adj_grad, &c->trk_vtx, &c->trk_type,
                     &c->trk_seq, &c->trk_count, &c->trk_pos})
     if (b->p) (void)hipFree(b->p);
@@ -3240,6 +3326,61 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   c->njobs = njobs;
   HIP_TRY(hipMemsetAsync(c->tallies.p, 0, tally_bytes, c->stream));
   HIP_TRY(hipMemsetAsync(c->counters.p, 0, sizeof(Counters) + 4 * sizeof(unsigned), c->stream));
+  // per-primitive log: room for one record per realisation, at most
+  // LOG_MAX_RECORDS (3.2 GB); none from the scratch array
+  c->T.log_key = nullptr;
+  c->T.log_val = nullptr;
+  c->T.log_n = nullptr;
+  c->T.log_cap = 0;
+  if (!arena && c->nprim) {
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(N, LOG_MAX_RECORDS);
+    uint32_t bits = 1;
+    while (bits < 32 && (uint64_t(1) << bits) < (uint64_t)c->nprim * 2) ++bits;
+    size_t sort_bytes = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint32_t*)nullptr,
+                                               (uint32_t*)nullptr, (const uint32_t*)nullptr,
+                                               (uint32_t*)nullptr, (int)cap, 0, (int)bits,
+                                               c->stream));
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t k4 = up((size_t)cap * 4);
+    const size_t bytes = 4 * k4 + up((size_t)cap * 80) + 256 + up(sort_bytes);
+    if ((r = alloc(c->plog, bytes))) return r;
+    char* q = (char*)c->plog.p;
+    c->T.log_key = (uint32_t*)q;
+    c->log_skey = (uint32_t*)(q + k4);
+    c->log_idx = (uint32_t*)(q + 2 * k4);
+    c->log_sidx = (uint32_t*)(q + 3 * k4);
+    c->T.log_val = (double*)(q + 4 * k4);
+    c->T.log_n = (unsigned long long*)(q + 4 * k4 + up((size_t)cap * 80));
+    c->log_sort = q + 4 * k4 + up((size_t)cap * 80) + 256;
+    c->log_sort_bytes = sort_bytes;
+    c->log_key_bits = bits;
+    c->T.log_cap = cap;
+    HIP_TRY(hipMemsetAsync(c->T.log_n, 0, sizeof(unsigned long long), c->stream));
+  }
+  // Sums the logged records into the per-primitive tally and empties the
+  // log (outside graph capture: the stream is the solve's)
+  auto log_flush = [&]() -> int {
+    if (!c->T.log_cap) return 0;
+    unsigned long long n = 0;
+    HIP_TRY(hipMemcpyAsync(&n, c->T.log_n, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const uint32_t m = (uint32_t)std::min<unsigned long long>(n, c->T.log_cap);
+    if (m) {
+      k_iota32<<<(m + 255) / 256, 256, 0, c->stream>>>(c->log_idx, m);
+      HIP_TRY(hipGetLastError());
+      size_t sb = c->log_sort_bytes;
+      HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->log_sort, sb, c->T.log_key, c->log_skey,
+                                                 c->log_idx, c->log_sidx, (int)m, 0,
+                                                 (int)c->log_key_bits, c->stream));
+      k_prim_reduce<<<(m + 255) / 256, 256, 0, c->stream>>>(c->log_skey, c->log_sidx,
+                                                           c->T.log_val, m, c->T.prim,
+                                                           c->T.mask_p, c->T.nprim);
+      HIP_TRY(hipGetLastError());
+    }
+    if (n) HIP_TRY(hipMemsetAsync(c->T.log_n, 0, sizeof(unsigned long long), c->stream));
+    return 0;
+  };
 
   SolveParams prm;
   prm.N = N;
@@ -3431,7 +3572,10 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       HIP_TRY(hipEventRecord(c->ev[1], c->stream));
       if (int e = iteration(cur, (count + BLOCK - 1) / BLOCK)) return e;
       unsigned total = 0;
+      unsigned long long logged = 0;
       HIP_TRY(hipMemcpyAsync(&total, cnt + (cur ^ 1), sizeof(total), hipMemcpyDeviceToHost, c->stream));
+      if (c->T.log_cap)
+        HIP_TRY(hipMemcpyAsync(&logged, c->T.log_n, sizeof(logged), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
       float t1 = 0, t2 = 0;
       (void)hipEventElapsedTime(&t1, c->ev[1], c->ev[2]);
@@ -3446,6 +3590,9 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       count = total;
       ++iter;
       ++wave_iter;
+      // past half full, the log is summed before it overflows into atomics
+      if (count && c->T.log_cap && logged >= c->T.log_cap / 2)
+        if (int e = log_flush()) return e;
     }
     return 0;
   };
@@ -3522,6 +3669,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       }
       gerr = drain_graph(count);
     }
+    if (!gerr) gerr = log_flush();
     if (!chunked || gerr) break;  // refill: one wavefront runs every realisation
     // Too many failed paths: like the reference's loop, which skips every
     // realisation not yet started once the limit is hit

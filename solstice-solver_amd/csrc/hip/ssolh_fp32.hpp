@@ -784,14 +784,7 @@ __device__ __forceinline__ void shade32_block(
       }
     }
     if (!__ballot(key != NONE)) break;
-    if (done && key != NONE) {
-      double* pb = T.prim + (size_t)(blockIdx.x & T.mask_p) * T.nprim * 40u;
-      for (int f = 0; f < 10; ++f) {
-        const double x = (double)a10[f] * fin.factor;
-        atomic_add(pb + (size_t)key * 20 + 2 * f, x);
-        atomic_add(pb + (size_t)key * 20 + 2 * f + 1, x * x);
-      }
-    }
+    prim_add(T, done && key != NONE, key, a10, fin.factor);
   }
 
   const bool want = live && fin.kind != FIN_NONE && !C->aborted;
