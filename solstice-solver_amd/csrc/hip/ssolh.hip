@@ -113,6 +113,11 @@ struct alignas(16) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 layout");
 
+struct StartAlias {
+  double prob;
+  uint32_t inst, inst_alias;
+};
+
 struct DScene {
   Spectra spectra;
   const ssolh_medium* media;
@@ -135,6 +140,9 @@ struct DScene {
   const IssInfo* iss;
   const ssolh_sample_tri* stris;
   const ssolh_alias* alias;
+  // per alias cell: its probability and the instances of its two sampled
+  // triangles (the start instance in one load, for k_start_keys)
+  const struct StartAlias* salias;
   uint32_t nstris;
   ssolh_sun sun;
   double S, S_proxy;
@@ -2116,98 +2124,63 @@ __global__ void k_iota(uint32_t* __restrict__ list, uint32_t n, unsigned* __rest
 // wave's rays leave neighbouring points in like directions (coherent
 // traversal) and its finished paths share tally keys (wave-reduced atomics).
 // The sun (job) comes first so a batch keeps each wave on one sun, as the
-// realisation order did. Three passes: k_start_rank draws each realisation's
-// start instance exactly as generate does and takes its rank in a histogram
-// over the keys; k_scan_small turns the histogram into offsets;
-// k_start_slots writes order[offset + rank] = q.
+// realisation order did. k_start_keys draws each realisation's start
+// instance exactly as generate does; a stable radix sort of (key, index)
+// pairs then gives the order, realisations of one key in index order.
 template <bool FP32>
 __device__ __forceinline__ uint32_t start_key(const DScene& S, const SolveParams& prm,
                                               uint64_t path, uint32_t ninst) {
-  uint32_t idx;
+  // S.stris[alias_pick(...)].instance (alias_pick32 for FP32) from one
+  // StartAlias load
+  uint32_t inst;
   if (FP32) {
     PathRng32 rng;
     rng.init(prm.key, prm.counter + path, 0);
     const float u0 = rng.next(), uc = rng.next();
-    idx = alias_pick32(S, u0, uc);
+    uint32_t i = (uint32_t)(u0 * (float)S.nstris);
+    if (i >= S.nstris) i = S.nstris - 1;
+    const StartAlias e = S.salias[i];
+    inst = (double)uc < e.prob ? e.inst : e.inst_alias;
   } else {
     PathRng rng;
     rng.init(prm.key, prm.counter + path, 0);
-    idx = alias_pick(S, rng.next());
+    const double x = rng.next() * (double)S.nstris;
+    uint32_t i = (uint32_t)x;
+    if (i >= S.nstris) i = S.nstris - 1;
+    const StartAlias e = S.salias[i];
+    inst = (x - (double)i) < e.prob ? e.inst : e.inst_alias;
   }
-  return job_of(prm, path) * ninst + S.stris[idx].instance;
+  return job_of(prm, path) * ninst + inst;
 }
 
-// Many keys: one global atomic per realisation (they rarely meet)
+// The start key of each of the wavefront's realisations and its index
 template <bool FP32>
 __global__ void __launch_bounds__(256)
-k_start_rank(DScene S, SolveParams prm, uint64_t first, uint32_t count, uint32_t ninst,
-             uint32_t* __restrict__ hist, uint2* __restrict__ key) {
+k_start_keys(DScene S, SolveParams prm, uint64_t first, uint32_t count, uint32_t ninst,
+             uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
   const uint32_t q = blockIdx.x * 256 + threadIdx.x;
   if (q >= count) return;
-  const uint32_t k = start_key<FP32>(S, prm, first + q, ninst);
-  key[q] = make_uint2(k, atomicAdd(hist + k, 1u));
+  key[q] = start_key<FP32>(S, prm, first + q, ninst);
+  idx[q] = q;
 }
 
-// Up to START_LDS keys: a block ranks per_block realisations (up to
-// START_PER_BLOCK; the host aims at about 1,024 blocks) in an LDS histogram,
-// then claims each key's range with one global atomic and adds that base to
-// its realisations' ranks. With many realisations per block a key's global
-// atomics drop from one per realisation to one per block (one heliostat, or
-// a few: otherwise all on one address or a few).
-constexpr uint32_t START_LDS = 16384;          // 64 KB of LDS
-constexpr uint32_t START_PER_BLOCK = 65536;
-template <bool FP32>
-__global__ void __launch_bounds__(256)
-k_start_rank_lds(DScene S, SolveParams prm, uint64_t first, uint32_t count, uint32_t ninst,
-                 uint32_t nkeys, uint32_t per_block, uint32_t* __restrict__ hist,
-                 uint2* __restrict__ key) {
-  __shared__ uint32_t h[START_LDS];
-  for (uint32_t i = threadIdx.x; i < nkeys; i += 256) h[i] = 0;
-  __syncthreads();
-  const uint32_t q0 = blockIdx.x * per_block;
-  const uint32_t q1 = min(count, q0 + per_block);
-  for (uint32_t q = q0 + threadIdx.x; q < q1; q += 256) {
-    const uint32_t k = start_key<FP32>(S, prm, first + q, ninst);
-    key[q] = make_uint2(k, atomicAdd(h + k, 1u));
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < nkeys; i += 256) {
-    const uint32_t n = h[i];
-    h[i] = n ? atomicAdd(hist + i, n) : 0u;
-  }
-  __syncthreads();
-  for (uint32_t q = q0 + threadIdx.x; q < q1; q += 256) {
-    uint2 v = key[q];
-    v.y += h[v.x];
-    key[q] = v;
-  }
+// Wavefronts are run in chunks of realisations, each ordered by start key,
+// when there are at most this many keys (more: slots are refilled)
+constexpr uint32_t CHUNK_KEYS = 16384;
+
+// Bits of the start keys (below nkeys) and the sort's work bytes for count
+// realisations
+uint32_t start_key_bits(uint64_t nkeys) {
+  uint32_t bits = 1;
+  while (bits < 32 && (uint64_t(1) << bits) < nkeys) ++bits;
+  return bits;
 }
 
-// Exclusive scan of a[0, n) in place by one block (a segment per thread)
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_small(uint32_t* __restrict__ a, uint32_t n) {
-  __shared__ uint32_t part[SCAN_THREADS];
-  const uint32_t per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
-  const uint32_t lo = min(n, threadIdx.x * per), hi = min(n, lo + per);
-  uint32_t s = 0;
-  for (uint32_t i = lo; i < hi; ++i) s += a[i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  block_inclusive_scan(part);
-  uint32_t run = part[threadIdx.x] - s;
-  for (uint32_t i = lo; i < hi; ++i) {
-    const uint32_t v = a[i];
-    a[i] = run;
-    run += v;
-  }
-}
-
-__global__ void __launch_bounds__(256)
-k_start_slots(const uint2* __restrict__ key, const uint32_t* __restrict__ offset, uint32_t count,
-              uint32_t* __restrict__ order) {
-  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
-  if (q >= count) return;
-  const uint2 k = key[q];
-  order[offset[k.x] + k.y] = q;
+hipError_t start_sort_bytes(uint32_t count, uint32_t bits, hipStream_t stream, size_t& bytes) {
+  bytes = 0;
+  return hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr,
+                                            (uint32_t*)nullptr, (const uint32_t*)nullptr,
+                                            (uint32_t*)nullptr, (int)count, 0, (int)bits, stream);
 }
 
 // Sum of the stripes of a striped tally into stripe 0
@@ -2352,7 +2325,7 @@ struct ssolh_context {
   DevBuf lbvh;
   DevBuf render;  // renderer work buffer (ssolh_render)
   // solve buffers
-  DevBuf start_hist;  // per-instance histogram / offsets of the first realisations
+  DevBuf start_sort;  // work bytes of the start-key sort of the first realisations
   DevBuf state, rays, hits, lists[2], emitted, block_emits, block_offsets, scan_chunks, counters,
       tallies, jobs, trace_ovf;
   // per-primitive log: keys, sorted keys, indices, sorted indices, values,
@@ -2688,7 +2661,7 @@ SSOLH_API int ssolh_destroy(ssolh_context* c) {
   free_scene(c);
   for (DevBuf* b : {&c->state, &c->rays, &c->hits, &c->lists[0], &c->lists[1], &c->emitted,
                     &c->block_emits, &c->block_offsets, &c->scan_chunks, &c->counters, &c->tallies,
-                    &c->jobs, &c->start_hist, &c->trace_ovf, &c->lbvh, &c->render, &c->plog,
+                    &c->jobs, &c->start_sort, &c->trace_ovf, &c->lbvh, &c->render, &c->plog,
                     &c->adj_state, &c->adj_in, &c->adj_grad, &c->trk_vtx, &c->trk_type,
                     &c->trk_seq, &c->trk_count, &c->trk_pos})
     if (b->p) (void)hipFree(b->p);
@@ -2838,6 +2811,11 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
   r |= upload(c, shapeq.data(), shapeq.size(), &c->shapeq);
   r |= upload(c, s->sample_tris, s->nsample_tris, &d.stris);
   r |= upload(c, s->sample_alias, s->nsample_tris, &d.alias);
+  std::vector<StartAlias> salias(s->nsample_tris);
+  for (uint32_t i = 0; i < s->nsample_tris; ++i)
+    salias[i] = {s->sample_alias[i].prob, s->sample_tris[i].instance,
+                 s->sample_tris[s->sample_alias[i].alias].instance};
+  r |= upload(c, salias.data(), salias.size(), &d.salias);
   if (r) return r;
   d.air = s->air_medium;
   d.tlas_root = troot;
@@ -3147,7 +3125,11 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   if (!slots64 && arena) {
     // from the scratch array: what is left beside the tallies, the
     // per-block scan arrays, counters and start-instance histogram
-    const size_t fixed = tally_bytes + (size_t)c->ninstances * sizeof(uint32_t) + (64u << 10);
+    size_t sort_bytes = 0;
+    const uint64_t most = std::min<uint64_t>(std::min<uint64_t>(N, arena->size / per_slot), 1u << 28);
+    HIP_TRY(start_sort_bytes((uint32_t)most, start_key_bits((uint64_t)njobs * c->ninstances),
+                             c->stream, sort_bytes));
+    const size_t fixed = tally_bytes + (sort_bytes + 255) / 256 * 256 + (64u << 10);
     const size_t rest = arena->size > fixed ? arena->size - fixed : 0;
     slots64 = rest / (per_slot + 1) / BLOCK * BLOCK;  // + 8 B of per-block counts per block
     if (slots64 < BLOCK) {
@@ -3406,15 +3388,14 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   uint32_t* scan_chunks = (uint32_t*)c->scan_chunks.p;
 
   // With fewer slots than realisations and something to order by (more than
-  // one sun or sampled instance) that ranks in LDS (at most START_LDS keys),
-  // the solve runs in chunks of `slots` realisations, each all in flight and
-  // in start-instance order, instead of refilling slots in place
+  // one sun or sampled instance) over at most CHUNK_KEYS keys, the solve runs
+  // in chunks of `slots` realisations, each all in flight and in
+  // start-instance order, instead of refilling slots in place
   // (100-paraboloid field at 2^20 slots: 449 -> 629 Mpaths/s). With more
-  // keys the ranking of every chunk costs more than the order gains (512-sun
-  // sweep, 51,712 keys: 1,483 -> 1,200), and slots are refilled.
+  // keys (512-sun sweep, 51,712 keys) slots are refilled.
   const uint64_t nkeys = (uint64_t)njobs * c->ninstances;
   const bool orderable = (uint64_t)njobs * c->nsamp > 1 && nkeys <= (1u << 24);
-  const bool chunked = slots < N && orderable && nkeys <= START_LDS;
+  const bool chunked = slots < N && orderable && nkeys <= CHUNK_KEYS;
 
   // one iteration: trace, shade, compaction of list i into list i^1
   auto iteration = [&](int i, uint32_t blocks) -> int {
@@ -3509,37 +3490,31 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   // slots [0, count) (list 0 is 0..count-1): in start-instance order when
   // no slot is refilled (a chunk, or every realisation in flight; refilled
   // slots take realisations r + slots, which in id order stay on one sun per
-  // wave and in this order would not). k_start_rank puts the rank and
-  // instance of each in the hit records and the order in the second work
-  // list, neither of which is used before the first trace.
+  // wave and in this order would not). The keys and indices before and
+  // after the sort take the hit records, the order the second work list,
+  // neither of which is used before the first trace.
   auto start = [&](uint64_t first, uint32_t count) -> int {
     k_iota<<<(count + 255) / 256, 256, 0, c->stream>>>(lists[0], count, cnt);
     HIP_TRY(hipGetLastError());
     const uint32_t* order = nullptr;
     if (count && (chunked || first + count == N) && orderable) {
-      const uint32_t nk = (uint32_t)nkeys, ni = c->ninstances;
-      if (int e = alloc(c->start_hist, (size_t)nk * sizeof(uint32_t))) return e;
-      uint32_t* hist = (uint32_t*)c->start_hist.p;
-      uint2* key = (uint2*)c->hits.p;
-      HIP_TRY(hipMemsetAsync(hist, 0, (size_t)nk * sizeof(uint32_t), c->stream));
-      const uint32_t per_block = std::min<uint32_t>(
-          START_PER_BLOCK, std::max<uint32_t>(4096, (count / 1024 + 255) / 256 * 256));
-      const uint32_t gl = (count + per_block - 1) / per_block, gg = (count + 255) / 256;
-      if (nk <= START_LDS && fp32)
-        k_start_rank_lds<true><<<gl, 256, 0, c->stream>>>(c->scn, prm, first, count, ni, nk,
-                                                          per_block, hist, key);
-      else if (nk <= START_LDS)
-        k_start_rank_lds<false><<<gl, 256, 0, c->stream>>>(c->scn, prm, first, count, ni, nk,
-                                                           per_block, hist, key);
-      else if (fp32)
-        k_start_rank<true><<<gg, 256, 0, c->stream>>>(c->scn, prm, first, count, ni, hist, key);
+      const uint32_t bits = start_key_bits(nkeys), gg = (count + 255) / 256;
+      size_t sort_bytes = 0;
+      HIP_TRY(start_sort_bytes(count, bits, c->stream, sort_bytes));
+      if (int e = alloc(c->start_sort, sort_bytes)) return e;
+      const size_t part = (size_t)(count + 63) / 64 * 64;  // 256-byte aligned parts
+      uint32_t* key = (uint32_t*)c->hits.p;
+      uint32_t* skey = key + part;
+      uint32_t* idx = skey + part;
+      if (fp32)
+        k_start_keys<true><<<gg, 256, 0, c->stream>>>(c->scn, prm, first, count, c->ninstances,
+                                                       key, idx);
       else
-        k_start_rank<false><<<gg, 256, 0, c->stream>>>(c->scn, prm, first, count, ni, hist, key);
+        k_start_keys<false><<<gg, 256, 0, c->stream>>>(c->scn, prm, first, count, c->ninstances,
+                                                        key, idx);
       HIP_TRY(hipGetLastError());
-      k_scan_small<<<1, SCAN_THREADS, 0, c->stream>>>(hist, nk);
-      HIP_TRY(hipGetLastError());
-      k_start_slots<<<gg, 256, 0, c->stream>>>(key, hist, count, lists[1]);
-      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->start_sort.p, sort_bytes, key, skey, idx,
+                                                 lists[1], (int)count, 0, (int)bits, c->stream));
       order = lists[1];
     }
     const uint32_t gb = (count + BLOCK - 1) / BLOCK;
@@ -3753,7 +3728,7 @@ SSOLH_API int ssolh_solve_noalloc(ssolh_context* c, const struct ssolh_solve_arg
   // put back after it.
   DevBuf* bufs[] = {&c->state,        &c->rays,         &c->hits,        &c->lists[0],
                     &c->lists[1],     &c->emitted,      &c->block_emits, &c->block_offsets,
-                    &c->scan_chunks,  &c->counters,     &c->tallies,     &c->start_hist,
+                    &c->scan_chunks,  &c->counters,     &c->tallies,     &c->start_sort,
                     &c->trace_ovf};
   constexpr size_t NB = sizeof(bufs) / sizeof(bufs[0]);
   DevBuf saved[NB];
