@@ -2164,10 +2164,6 @@ k_start_keys(DScene S, SolveParams prm, uint64_t first, uint32_t count, uint32_t
   idx[q] = q;
 }
 
-// Wavefronts are run in chunks of realisations, each ordered by start key,
-// when there are at most this many keys (more: slots are refilled)
-constexpr uint32_t CHUNK_KEYS = 16384;
-
 // Bits of the start keys (below nkeys) and the sort's work bytes for count
 // realisations
 uint32_t start_key_bits(uint64_t nkeys) {
@@ -3388,14 +3384,13 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   uint32_t* scan_chunks = (uint32_t*)c->scan_chunks.p;
 
   // With fewer slots than realisations and something to order by (more than
-  // one sun or sampled instance) over at most CHUNK_KEYS keys, the solve runs
-  // in chunks of `slots` realisations, each all in flight and in
-  // start-instance order, instead of refilling slots in place
-  // (100-paraboloid field at 2^20 slots: 449 -> 629 Mpaths/s). With more
-  // keys (512-sun sweep, 51,712 keys) slots are refilled.
+  // one sun or sampled instance), the solve runs in chunks of `slots`
+  // realisations, each all in flight and in start-instance order, instead of
+  // refilling slots in place (100-paraboloid field at 2^20 slots: 449 -> 629
+  // Mpaths/s; its 512-sun sweep, 51,712 keys: 1,680 -> 2,290).
   const uint64_t nkeys = (uint64_t)njobs * c->ninstances;
   const bool orderable = (uint64_t)njobs * c->nsamp > 1 && nkeys <= (1u << 24);
-  const bool chunked = slots < N && orderable && nkeys <= CHUNK_KEYS;
+  const bool chunked = slots < N && orderable;
 
   // one iteration: trace, shade, compaction of list i into list i^1
   auto iteration = [&](int i, uint32_t blocks) -> int {
