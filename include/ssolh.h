@@ -511,6 +511,17 @@ SSOLH_API int ssolh_test_trace_stack(ssolh_context* ctx, uint32_t lds_cap, uint3
  * rays cross near their faces. */
 SSOLH_API int ssolh_test_slab_slack(ssolh_context* ctx, float scale);
 
+/* Log of per-primitive entries (finished paths append their entries; the
+ * log is sorted by key and summed into the per-primitive tally after each
+ * wavefront and whenever a host-driven iteration leaves it half full; a
+ * full log sends further entries to per-lane atomics): caps it at `records`
+ * entries for the solves that follow (0 restores the default, one per
+ * realisation up to 2^25). *flushes and *overflowed (if not NULL) = the last
+ * solve's sorted reductions and the entries it sent to the atomics. The
+ * results do not depend on the cap beyond the order of the additions. */
+SSOLH_API int ssolh_test_prim_log(ssolh_context* ctx, uint32_t records, uint32_t* flushes,
+                                  uint64_t* overflowed);
+
 /* Scene uploads, TLAS refits and GPU TLAS builds this context has done */
 SSOLH_API int ssolh_test_scene_counters(ssolh_context* ctx, uint32_t* uploads, uint32_t* refits,
                                         uint32_t* builds);

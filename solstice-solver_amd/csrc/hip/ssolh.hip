@@ -2332,6 +2332,9 @@ struct ssolh_context {
   size_t log_sort_bytes = 0;
   uint32_t *log_skey = nullptr, *log_idx = nullptr, *log_sidx = nullptr;
   uint32_t log_key_bits = 1;
+  uint32_t log_cap_test = 0;          // ssolh_test_prim_log's cap (0: the default)
+  uint32_t log_flushes = 0;           // the last solve's sorted reductions
+  uint64_t log_overflowed = 0;        // and the entries it sent to the atomics
   DevBuf adj_state, adj_in, adj_grad;
   DevBuf trk_vtx, trk_type, trk_seq, trk_count, trk_pos;
   uint64_t trk_paths = 0;           // realisations of the last tracked solve
@@ -3310,8 +3313,11 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   c->T.log_val = nullptr;
   c->T.log_n = nullptr;
   c->T.log_cap = 0;
+  c->log_flushes = 0;
+  c->log_overflowed = 0;
   if (!arena && c->nprim) {
-    const uint32_t cap = (uint32_t)std::min<uint64_t>(N, LOG_MAX_RECORDS);
+    uint32_t cap = (uint32_t)std::min<uint64_t>(N, LOG_MAX_RECORDS);
+    if (c->log_cap_test) cap = std::min(cap, c->log_cap_test);
     uint32_t bits = 1;
     while (bits < 32 && (uint64_t(1) << bits) < (uint64_t)c->nprim * 2) ++bits;
     size_t sort_bytes = 0;
@@ -3344,7 +3350,9 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     HIP_TRY(hipMemcpyAsync(&n, c->T.log_n, sizeof(n), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     const uint32_t m = (uint32_t)std::min<unsigned long long>(n, c->T.log_cap);
+    c->log_overflowed += n - m;
     if (m) {
+      ++c->log_flushes;
       k_iota32<<<(m + 255) / 256, 256, 0, c->stream>>>(c->log_idx, m);
       HIP_TRY(hipGetLastError());
       size_t sb = c->log_sort_bytes;
@@ -3985,6 +3993,15 @@ SSOLH_API int ssolh_test_trace_stack(ssolh_context* c, uint32_t lds_cap, uint32_
   }
   c->lds_stack_cap = lds_cap;
   if (bound) *bound = c->has_scene ? c->stack_depth : 0;
+  return 0;
+}
+
+SSOLH_API int ssolh_test_prim_log(ssolh_context* c, uint32_t records, uint32_t* flushes,
+                                  uint64_t* overflowed) {
+  if (!c) return SSOLH_ERR_ARG;
+  c->log_cap_test = records;
+  if (flushes) *flushes = c->log_flushes;
+  if (overflowed) *overflowed = c->log_overflowed;
   return 0;
 }
 

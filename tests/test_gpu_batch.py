@@ -63,7 +63,7 @@ def test_graph_mode_matches_host_loop(gpu):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_graph_replayed_many_times_matches_host_loop(gpu, precision):
-    """A narrow wavefront (4,096 slots refilled in place) with two
+    """A narrow wavefront (chunks of 4,096 start-ordered realisations) with two
     iterations per graph: each solve replays its captured graph dozens of
     times, with a device-to-host copy and a synchronisation between replays,
     and three solves in a row capture it anew; every solve equals the

@@ -2,10 +2,9 @@
 realisation of a solve is in flight at once; with a narrower width
 (ssol_device_set_wavefront_size) the solve either runs in chunks of that
 many realisations, each all in flight (when there is something to order the
-slots by: more than one sampled instance or sun, with at most 16,384
-(sun, instance) keys), or keeps one wavefront whose slots take further
-realisations in place, which runs the kernels' refill code (one sampled
-instance and one sun, or more keys than that). Realisation ids are static,
+slots by: more than one sampled instance or sun), or keeps one wavefront
+whose slots take further realisations in place, which runs the kernels'
+refill code (one sampled instance and one sun). Realisation ids are static,
 so both runs follow the same paths; only the order of the FP64 tally
 additions may differ (tolerance 1e-10 relative). Covers the plain, FP32,
 adjoint, tracked and batched solves in both narrow forms."""
@@ -140,9 +139,9 @@ def test_narrow_wavefront_refill_tracked_paths(gpu):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_narrow_wavefront_refill_batch_many_keys(gpu, precision):
-    """800 suns x 21 instances = 16,800 (sun, instance) keys, above what
-    ranks in LDS: the narrow batch refills slots in place across suns."""
+def test_narrow_wavefront_batch_many_keys(gpu, precision):
+    """800 suns x 21 instances = 16,800 (sun, instance) keys: the narrow
+    batch runs in chunks of start-ordered realisations that span suns."""
     spec = S.config2(0, n_helio=20, radius=150.0)
     rng = np.random.default_rng(8)
     dirs = [S.sun_dir_from(a, e) for a, e in zip(rng.uniform(0, 360, 800), rng.uniform(20, 80, 800))]
