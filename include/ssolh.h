@@ -516,7 +516,7 @@ SSOLH_API int ssolh_test_slab_slack(ssolh_context* ctx, float scale);
  * wavefront and whenever a host-driven iteration leaves it half full; a
  * full log sends further entries to per-lane atomics): caps it at `records`
  * entries for the solves that follow (0 restores the default, one per
- * realisation up to 2^25). *flushes and *overflowed (if not NULL) = the last
+ * realisation up to 2^27). *flushes and *overflowed (if not NULL) = the last
  * solve's sorted reductions and the entries it sent to the atomics. The
  * results do not depend on the cap beyond the order of the additions. */
 SSOLH_API int ssolh_test_prim_log(ssolh_context* ctx, uint32_t records, uint32_t* flushes,

@@ -2278,7 +2278,7 @@ struct DevBuf {
 // device current (kept per device by ssolh_create).
 constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
 // records of the per-primitive log (84 B each plus 12 B of sort arrays)
-constexpr uint64_t LOG_MAX_RECORDS = uint64_t(1) << 25;
+constexpr uint64_t LOG_MAX_RECORDS = uint64_t(1) << 27;
 static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
 
 constexpr int MAX_DEVICES = 64;
@@ -3308,7 +3308,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   HIP_TRY(hipMemsetAsync(c->tallies.p, 0, tally_bytes, c->stream));
   HIP_TRY(hipMemsetAsync(c->counters.p, 0, sizeof(Counters) + 4 * sizeof(unsigned), c->stream));
   // per-primitive log: room for one record per realisation, at most
-  // LOG_MAX_RECORDS (3.2 GB); none from the scratch array
+  // LOG_MAX_RECORDS (12.9 GB); none from the scratch array
   c->T.log_key = nullptr;
   c->T.log_val = nullptr;
   c->T.log_n = nullptr;
@@ -3328,19 +3328,26 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t k4 = up((size_t)cap * 4);
     const size_t bytes = 4 * k4 + up((size_t)cap * 80) + 256 + up(sort_bytes);
-    if ((r = alloc(c->plog, bytes))) return r;
-    char* q = (char*)c->plog.p;
-    c->T.log_key = (uint32_t*)q;
-    c->log_skey = (uint32_t*)(q + k4);
-    c->log_idx = (uint32_t*)(q + 2 * k4);
-    c->log_sidx = (uint32_t*)(q + 3 * k4);
-    c->T.log_val = (double*)(q + 4 * k4);
-    c->T.log_n = (unsigned long long*)(q + 4 * k4 + up((size_t)cap * 80));
-    c->log_sort = q + 4 * k4 + up((size_t)cap * 80) + 256;
-    c->log_sort_bytes = sort_bytes;
-    c->log_key_bits = bits;
-    c->T.log_cap = cap;
-    HIP_TRY(hipMemsetAsync(c->T.log_n, 0, sizeof(unsigned long long), c->stream));
+    if (alloc(c->plog, bytes)) {
+      // no room left beside the path state: every entry takes the atomics
+      (void)hipGetLastError();
+      c->plog = DevBuf{};
+      cap = 0;
+    }
+    if (cap) {
+      char* q = (char*)c->plog.p;
+      c->T.log_key = (uint32_t*)q;
+      c->log_skey = (uint32_t*)(q + k4);
+      c->log_idx = (uint32_t*)(q + 2 * k4);
+      c->log_sidx = (uint32_t*)(q + 3 * k4);
+      c->T.log_val = (double*)(q + 4 * k4);
+      c->T.log_n = (unsigned long long*)(q + 4 * k4 + up((size_t)cap * 80));
+      c->log_sort = q + 4 * k4 + up((size_t)cap * 80) + 256;
+      c->log_sort_bytes = sort_bytes;
+      c->log_key_bits = bits;
+      c->T.log_cap = cap;
+      HIP_TRY(hipMemsetAsync(c->T.log_n, 0, sizeof(unsigned long long), c->stream));
+    }
   }
   // Sums the logged records into the per-primitive tally and empties the
   // log (outside graph capture: the stream is the solve's)
