@@ -84,6 +84,10 @@ void set_err(const char* fmt, ...) {
 // -------------------------------------------------------------------------
 // Device scene
 // -------------------------------------------------------------------------
+// IssInfo::obj_flags bit (beside the SSOLH_OBJECT_* ones): the object has
+// one shaded shape
+constexpr uint32_t OBJ_ONE_SHAPE = 1u << 31;
+
 // What a hit on (instance, shaded shape) needs, gathered at upload so that a
 // hit costs one lookup: global index = instance.isshape_first + sshape.
 struct alignas(8) IssInfo {
@@ -93,7 +97,7 @@ struct alignas(8) IssInfo {
   int32_t receiver;        // receiver slot of the instance or -1
   uint32_t receiver_mask;  // ssol_side_flag bits
   uint32_t shape_type;     // enum ssolh_shape_type
-  uint32_t obj_flags;      // SSOLH_OBJECT_* of the instance's object
+  uint32_t obj_flags;      // SSOLH_OBJECT_* of the instance's object, OBJ_ONE_SHAPE
   int64_t prim_tally;      // first per-primitive slot or -1
 };
 
@@ -143,6 +147,7 @@ struct DScene {
   // per alias cell: its probability and the instances of its two sampled
   // triangles (the start instance in one load, for k_start_keys)
   const struct StartAlias* salias;
+  const float* obj_box;  // [object][6]: object-space lo xyz, hi xyz
   uint32_t nstris;
   ssolh_sun sun;
   double S, S_proxy;
@@ -657,6 +662,60 @@ __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t is
   return true;
 }
 
+// Distance at which the world ray (o, d) leaves the object box of instance
+// inst_id (in the instance's object space, where the trace's triangle
+// distances are taken); an axis the ray runs parallel to bounds nothing.
+__device__ double object_exit(const DScene& S, uint32_t inst_id, d3 o, d3 d) {
+  const ssolh_instance& inst = S.insts[inst_id];
+  const d3 T = mk(inst.transform[9], inst.transform[10], inst.transform[11]);
+  const d3 lo = mul(inst.inv_linear, o - T), ld = mul(inst.inv_linear, d);
+  const float* b = S.obj_box + 6 * (size_t)inst.object;
+  const double oo[3] = {lo.x, lo.y, lo.z}, dd[3] = {ld.x, ld.y, ld.z};
+  double t = INFINITY;
+  for (int a = 0; a < 3; ++a) {
+    if (dd[a] == 0) continue;
+    const double t0 = ((double)b[a] - oo[a]) / dd[a], t1 = ((double)b[3 + a] - oo[a]) / dd[a];
+    t = fmin(t, fmax(t0, t1));
+  }
+  return t;
+}
+
+// Whether the filter can accept a candidate on (instance, shaded shape)
+// iss_id, a punched shape, for a ray that leaves it at tmin 0 (world ray o,
+// d) and whose candidates on it lie within distance t_hi. The filter
+// refines every candidate of the shape to the root of the same ray /
+// quadric equation nearest the candidate's distance, and accepts it only
+// past tmin and on the side the ray left. A root that passes these checks
+// (computed here exactly as the filter computes them) can be chosen when it
+// is the only root, when it is the nearer one, or when the candidates reach
+// the midpoint between the roots; otherwise no candidate of the shape can be
+// accepted.
+__device__ bool punched_rehit_possible(const DScene& S, const RayRec& r, d3 o, d3 d,
+                                       uint32_t iss_id, double t_hi) {
+  const uint32_t side_from = (r.flags >> RF_SIDE_SHIFT) & 3u;
+  const IssInfo is = S.iss[iss_id];
+  const ssolh_inst_sshape& q = S.isshapes[iss_id];
+  const ssolh_shape& sh = S.shapes[is.shape];
+  const d3 lo = mul(q.R_inv, o - ld3(q.T));
+  const d3 ldir = mul(q.R_inv, d);
+  double a, b, c, t[2];
+  if (!quadric_coeffs(sh.quadric, sh.qparam, lo, ldir, a, b, c)) return false;
+  const int n = quadric_roots(a, b, c, t);
+  for (int k = 0; k < n; ++k) {
+    if (!(t[k] < (double)FLT_MAX)) continue;
+    if ((double)(float)t[k] <= r.tmin) continue;
+    const d3 grad = quadric_grad(sh.quadric, sh.qparam, lo + ldir * t[k]);
+    const d3 qn = normalize(tmul(q.R_inv, grad));
+    const uint32_t side = dot(d, qn) < 0 ? SSOL_FRONT_ : SSOL_BACK_;
+    if (side != side_from) continue;
+    if (n == 1) return true;
+    const double other = t[1 - k];
+    if (!(fabs(other) < (double)FLT_MAX) || t[k] <= other) return true;
+    if (!(t_hi * 1.001 + 1e-3 < 0.5 * (t[k] + other))) return true;
+  }
+  return false;
+}
+
 // Unit world normal of the quadric of (instance, shaded shape) `iss` where the
 // world ray (o, d) meets it at distance dst: the gradient of quadric_hit's
 // refinement at the same point, as the filter took it
@@ -821,10 +880,15 @@ __device__ __forceinline__ void trace_ray(const DScene& S, const uint32_t* __res
   const float ftmin = r.tmin;
   // A ray that leaves a point of a planar object can meet that object again
   // only at t ~ 0, which the filter rejects: its instance is not entered.
+  // Nor is the instance of a one-shape object whose punched shape the ray
+  // leaves when the filter could accept no candidate on it.
   uint32_t skip_inst = NONE;
   if (r.tmin == 0.f) {
     const IssInfo e = S.iss[r.iss_from];
-    if (e.obj_flags & SSOLH_OBJECT_PLANAR) skip_inst = e.inst;
+    if ((e.obj_flags & SSOLH_OBJECT_PLANAR) ||
+        ((e.obj_flags & OBJ_ONE_SHAPE) && e.shape_type != SSOLH_SHAPE_MESH &&
+         !punched_rehit_possible(S, r, ow, dw, r.iss_from, object_exit(S, e.inst, ow, dw))))
+      skip_inst = e.inst;
   }
 
   // "while-while" traversal: the lanes of a wave first descend through inner
@@ -2481,7 +2545,7 @@ int build_iss(const ssolh_scene* s, std::vector<IssInfo>& out) {
       d.receiver = in.receiver;
       d.receiver_mask = in.receiver_mask;
       d.shape_type = (uint32_t)s->shapes[sh.shape].type;
-      d.obj_flags = o.flags;
+      d.obj_flags = o.flags | (o.sshape_count == 1 ? OBJ_ONE_SHAPE : 0u);
       d.prim_tally = s->inst_sshapes[g].prim_tally;
     }
   }
@@ -2798,6 +2862,7 @@ SSOLH_API int ssolh_scene_upload(ssolh_context* c, const struct ssolh_scene* s) 
     r |= upload(c, &zero, 1, (const double**)&c->refit_cost);
   }
   r |= upload(c, s->object_boxes, (size_t)s->nobjects * 6, &c->obj_box);
+  d.obj_box = c->obj_box;
   // a lane's stack: the TLAS descent's pushes, the instance sentinel, the
   // deepest BLAS descent's pushes
   c->stack_depth = col.pb[troot] + 1 + blas_pb;

@@ -227,19 +227,28 @@ SD void basis(d3 n, d3& x, d3& y) {
 }
 
 // Quadrics (reference src/ssol_shape.c:236-285, 864-1135) ------------------
-SD int solve_second(double a, double b, double c, double hint, double& t) {
+// The real roots of a t^2 + b t + c (0, 1 or 2 of them)
+SD int quadric_roots(double a, double b, double c, double t[2]) {
   if (a == 0) {
-    if (b != 0) { t = -c / b; return 1; }
+    if (b != 0) { t[0] = -c / b; return 1; }
     return 0;
   }
   const double delta = b * b - 4 * a * c;
   if (delta < 0) return 0;  // the reference's NaN roots; never kept as a hit
-  if (delta == 0) { t = -b / (2 * a); return 1; }
+  if (delta == 0) { t[0] = -b / (2 * a); return 1; }
   const double sq = sqrt(delta);
-  const double t1 = (-b - copysign(sq, b)) / (2 * a);
-  const double t2 = c / (a * t1);
-  t = fabs(t1 - hint) < fabs(t2 - hint) ? t1 : t2;
+  t[0] = (-b - copysign(sq, b)) / (2 * a);
+  t[1] = c / (a * t[0]);
   return 2;
+}
+
+// The root nearest the hint
+SD int solve_second(double a, double b, double c, double hint, double& t) {
+  double r[2];
+  const int n = quadric_roots(a, b, c, r);
+  if (n == 0) return 0;
+  t = n == 1 || fabs(r[0] - hint) < fabs(r[1] - hint) ? r[0] : r[1];
+  return n;
 }
 
 SD double quadric_z(int q, const double* p, double x, double y) {
@@ -273,9 +282,9 @@ SD d3 quadric_grad(int q, const double* p, d3 pt) {
   }
 }
 
-// Ray / quadric in the quadric frame; returns 0 when there is no root.
-SD int quadric_hit(int q, const double* p, d3 o, d3 d, double hint, double& t, d3& grad) {
-  double a, b, c;
+// Coefficients of the ray / quadric equation a t^2 + b t + c = 0 in the
+// quadric frame; false for no quadric.
+SD bool quadric_coeffs(int q, const double* p, d3 o, d3 d, double& a, double& b, double& c) {
   switch (q) {
     case SSOLH_Q_PLANE: a = 0; b = d.z; c = o.z; break;
     case SSOLH_Q_PARABOL:
@@ -309,8 +318,16 @@ SD int quadric_hit(int q, const double* p, d3 o, d3 d, double hint, double& t, d
       c = o.x * o.x + o.y * o.y + o.z * o.z - p[1] + 2 * z0 * o.z + z0 * z0;
       break;
     }
-    default: return 0;
+    default: return false;
   }
+  return true;
+}
+
+// Ray / quadric in the quadric frame: the root nearest the hint and the
+// gradient there; returns 0 when there is no root.
+SD int quadric_hit(int q, const double* p, d3 o, d3 d, double hint, double& t, d3& grad) {
+  double a, b, c;
+  if (!quadric_coeffs(q, p, o, d, a, b, c)) return 0;
   if (!solve_second(a, b, c, hint, t)) return 0;
   grad = quadric_grad(q, p, o + d * t);
   return 1;
