@@ -662,15 +662,10 @@ __device__ bool filter(const DScene& S, const RayRec& r, d3 o, d3 d, uint32_t is
   return true;
 }
 
-// Distance at which the world ray (o, d) leaves the object box of instance
-// inst_id (in the instance's object space, where the trace's triangle
-// distances are taken); an axis the ray runs parallel to bounds nothing.
-__device__ double object_exit(const DScene& S, uint32_t inst_id, d3 o, d3 d) {
-  const ssolh_instance& inst = S.insts[inst_id];
-  const d3 T = mk(inst.transform[9], inst.transform[10], inst.transform[11]);
-  const d3 lo = mul(inst.inv_linear, o - T), ld = mul(inst.inv_linear, d);
-  const float* b = S.obj_box + 6 * (size_t)inst.object;
-  const double oo[3] = {lo.x, lo.y, lo.z}, dd[3] = {ld.x, ld.y, ld.z};
+// Distance at which the ray (o, d) leaves box b (lo xyz, hi xyz); an axis
+// the ray runs parallel to bounds nothing.
+__device__ double box_exit(const float* b, d3 o, d3 d) {
+  const double oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
   double t = INFINITY;
   for (int a = 0; a < 3; ++a) {
     if (dd[a] == 0) continue;
@@ -678,6 +673,26 @@ __device__ double object_exit(const DScene& S, uint32_t inst_id, d3 o, d3 d) {
     t = fmin(t, fmax(t0, t1));
   }
   return t;
+}
+
+// Distance at which the world ray (o, d) leaves the object box of instance
+// inst_id (in the instance's object space, where the trace's triangle
+// distances are taken)
+__device__ double object_exit(const DScene& S, uint32_t inst_id, d3 o, d3 d) {
+  const ssolh_instance& inst = S.insts[inst_id];
+  const d3 T = mk(inst.transform[9], inst.transform[10], inst.transform[11]);
+  return box_exit(S.obj_box + 6 * (size_t)inst.object, mul(inst.inv_linear, o - T),
+                  mul(inst.inv_linear, d));
+}
+
+// Whether root k of the n roots t[] of a ray / quadric equation can be the
+// one the filter's refinement takes for a candidate at a distance in
+// [0, t_hi]: the root nearest the candidate's distance
+__device__ bool root_choosable(const double t[2], int n, int k, double t_hi) {
+  if (n == 1) return true;
+  const double other = t[1 - k];
+  if (!(fabs(other) < (double)FLT_MAX) || t[k] <= other) return true;
+  return !(t_hi * 1.001 + 1e-3 < 0.5 * (t[k] + other));
 }
 
 // Whether the filter can accept a candidate on (instance, shaded shape)
@@ -708,10 +723,7 @@ __device__ bool punched_rehit_possible(const DScene& S, const RayRec& r, d3 o, d
     const d3 qn = normalize(tmul(q.R_inv, grad));
     const uint32_t side = dot(d, qn) < 0 ? SSOL_FRONT_ : SSOL_BACK_;
     if (side != side_from) continue;
-    if (n == 1) return true;
-    const double other = t[1 - k];
-    if (!(fabs(other) < (double)FLT_MAX) || t[k] <= other) return true;
-    if (!(t_hi * 1.001 + 1e-3 < 0.5 * (t[k] + other))) return true;
+    if (root_choosable(t, n, k, t_hi)) return true;
   }
   return false;
 }

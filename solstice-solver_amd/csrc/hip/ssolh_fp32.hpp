@@ -164,6 +164,33 @@ __device__ bool filter32(const DScene& S, const ShapeQ* SQ, const RayRec32& r, f
   return true;
 }
 
+// punched_rehit_possible for the FP32 form, with the FP32 filter's own
+// quadric frame and side test (ray rebased at base, candidates within
+// distance t_hi)
+__device__ bool punched_rehit_possible32(const DScene& S, const ShapeQ* __restrict__ SQ,
+                                         const RayRec32& r, uint32_t iss_id, double t_hi) {
+  const uint32_t side_from = (r.flags >> RF_SIDE_SHIFT) & 3u;
+  const IssInfo is = S.iss[iss_id];
+  const ssolh_shape& sh = S.shapes[is.shape];
+  const ShapeQ& q = SQ[is.shape];
+  d3 od, dd;
+  to_instance(S, r.base, mkf(r.ox, r.oy, r.oz), mkf(r.dx, r.dy, r.dz), is.inst, od, dd);
+  const d3 lo = mul(q.Rinv, od - ld3(q.t));
+  const d3 ld = mul(q.Rinv, dd);
+  double a, b, c, t[2];
+  if (!quadric_coeffs(sh.quadric, sh.qparam, lo, ld, a, b, c)) return false;
+  const int n = quadric_roots(a, b, c, t);
+  for (int k = 0; k < n; ++k) {
+    if (!(t[k] < (double)FLT_MAX)) continue;
+    if ((float)t[k] <= r.tmin) continue;
+    const d3 grad = quadric_grad(sh.quadric, sh.qparam, lo + ld * t[k]);
+    const uint32_t side = dot(ld, grad) < 0 ? SSOL_FRONT_ : SSOL_BACK_;
+    if (side != side_from) continue;
+    if (root_choosable(t, n, k, t_hi)) return true;
+  }
+  return false;
+}
+
 // Entry q of the work list, FP32 form
 template <int STACK>
 __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __restrict__ SQ,
@@ -191,11 +218,20 @@ __device__ __forceinline__ void trace_ray32(const DScene& S, const ShapeQ* __res
   FRay fr = make_fray32(ow.x, ow.y, ow.z, dw.x, dw.y, dw.z);
   const FRay fw = fr;
   FShear sh = make_shear32(dw);
-  // a ray leaving a planar object does not enter that object again
+  // a ray leaving a planar object does not enter that object again, nor a
+  // one-shape object's punched shape that no candidate could hit again
+  // (as trace_ray)
   uint32_t skip_inst = NONE;
   if (r.tmin == 0.f) {
     const IssInfo e = S.iss[r.iss_from];
-    if (e.obj_flags & SSOLH_OBJECT_PLANAR) skip_inst = e.inst;
+    bool skip = (e.obj_flags & SSOLH_OBJECT_PLANAR) != 0;
+    if (!skip && (e.obj_flags & OBJ_ONE_SHAPE) && e.shape_type != SSOLH_SHAPE_MESH) {
+      d3 od, dd;
+      to_instance(S, r.base, off, dw, e.inst, od, dd);
+      const double t_hi = box_exit(S.obj_box + 6 * (size_t)S.insts[e.inst].object, od, dd);
+      skip = !punched_rehit_possible32(S, SQ, r, r.iss_from, t_hi);
+    }
+    if (skip) skip_inst = e.inst;
   }
   bool in_blas = false;
   uint32_t iss_first = 0;

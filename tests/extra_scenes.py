@@ -240,6 +240,20 @@ def hemisphere_mirror(R_=10.0, r=1.0):
         hemisphere_proxy_area(r, n)
 
 
+def hemisphere_deep_mirror(R_=10.0, rho=0.8, zenith_deg=35.0):
+    """The whole hemisphere bowl as a mirror of reflectivity rho under a sun
+    zenith_deg off vertical: its rim shadows part of the bowl and reflected
+    rays meet the bowl again, most of them several times, before they leave
+    through the opening. The bowl is its own receiver on both sides, so its
+    incoming flux counts every hit."""
+    bowl = S.Punched(S.Q_HEMISPHERE, (R_, 0.0), list(S.IDENTITY), [], slices_hint=S.SIZE_MAX)
+    m = S.mirror(rho)
+    inst = S.Instance(S.Object([(bowl, m, m)]), S.IDENTITY, receiver_mask=S.FRONT | S.BACK)
+    z = math.radians(zenith_deg)
+    sun = S.Sun("directional", (math.sin(z), 0.0, -math.cos(z)), DNI, R.spectrum3())
+    return S.Scene([inst], sun, name="hemisphere_deep_mirror"), {"bowl": inst}
+
+
 def check_hemisphere_mirror(run, n, area, handles):
     assert run.res == 0 and run.realisation_count == n
     for t in ("t1", "t2"):

@@ -11,7 +11,9 @@ forms and against the oracle:
 - SUB, non-convex AND/SUB carvings and a mirror with holes
   (src/ssol_shape.c:579-652);
 - the hemisphere quadric (src/ssol_shape.c:386-435, 1089-1111): absorbing
-  bowls and a focusing mirror bowl.
+  bowls, a focusing mirror bowl and a deep mirror bowl whose rays meet it
+  again (the trace skips the instance of a punched shape a ray leaves only
+  where the filter could accept no candidate on it).
 """
 import pytest
 
@@ -147,3 +149,29 @@ def test_hemisphere_mirror_focus(gpu, precision):
     scn, h, area = X.hemisphere_mirror()
     run = R.Run("gpu32" if precision == "fp32" else "gpu", scn, 100_000, seed=5)
     X.check_hemisphere_mirror(run, 100_000, area, h)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_hemisphere_deep_mirror_rehits(gpu, precision):
+    """Rays leaving the bowl meet it again: shadow rays toward the sun are
+    stopped by the far rim, reflections hit the bowl several times. The GPU
+    agrees with the oracle, whose trace enters every instance it crosses,
+    on the bowl's incoming and absorbed flux (both sides) and the global
+    sinks."""
+    scn, h = X.hemisphere_deep_mirror()
+    n = 400_000
+    run = R.Run("gpu32" if precision == "fp32" else "gpu", scn, n, seed=6)
+    o = R.Run("oracle", scn, n, seed=6)
+    # a path traces its shadow ray, then (unless shadowed, a tenth of them)
+    # its first reflection, about 1.9 segments without re-hits
+    assert run.e.stats.segments > 2.3 * n
+    g, og = run.global_(), o.global_()
+    assert og["shadowed"][0] > 0.05 * og["absorbed_by_receivers"][0]   # the rim shadows
+    for k in ("shadowed", "missing", "absorbed_by_receivers"):
+        ok, tol = within(g[k], og[k], k=4.0)
+        assert ok, (k, g[k], og[k], tol)
+    for side in (X.S.FRONT, X.S.BACK):
+        rg, ro = run.receiver(h["bowl"], side), o.receiver(h["bowl"], side)
+        for f in ("incoming_flux", "absorbed_flux"):
+            ok, tol = within(rg[f], ro[f], k=4.0)
+            assert ok, (side, f, rg[f], ro[f], tol)
