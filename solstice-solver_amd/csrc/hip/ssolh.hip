@@ -424,46 +424,88 @@ __device__ void prim_add(const Tallies& T, bool has, uint32_t key, const R* a, d
   }
 }
 
-// The sorted log (keys skey, record indices sidx, m records): each wave sums
-// the runs of equal keys among its 64 records (x and x^2 of the 10 fields)
-// and adds each run with one set of 20 atomics.
+// The sorted log (keys skey, record indices sidx, m records): each wave
+// takes PRIM_ROUNDS rounds of 64 consecutive records. While a round's
+// records all carry the key of the sums the wave holds, each lane adds its
+// record's x and x^2 of the 10 fields to its own sums; at a change of key
+// the sums are reduced across the wave and added with one set of 20
+// atomics, and a round of several keys reduces each of its runs of equal
+// keys so.
+constexpr uint32_t PRIM_ROUNDS = 8;
+
+__device__ __forceinline__ void prim_flush_run(double* pb, uint32_t key, const double* acc,
+                                               int leader) {
+#pragma unroll
+  for (int f = 0; f < 20; ++f) {
+    const double s = wave_sum(acc[f]);
+    if ((int)lane_id() == leader) atomic_add(pb + (size_t)key * 20u + f, s);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_prim_reduce(const uint32_t* __restrict__ skey,
                                                      const uint32_t* __restrict__ sidx,
                                                      const double* __restrict__ val, uint32_t m,
                                                      double* prim, uint32_t mask_p, uint64_t nprim) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = i < m;
-  const uint32_t key = in ? skey[i] : 0u;
-  double x[10];
-  if (in) {
-    const double2* v = (const double2*)(val + (size_t)sidx[i] * 10u);
-#pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      const double2 t = v[f];
-      x[2 * f] = t.x;
-      x[2 * f + 1] = t.y;
-    }
-  } else {
-#pragma unroll
-    for (int f = 0; f < 10; ++f) x[f] = 0.0;
-  }
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
+  const uint32_t base = wave * 64u * PRIM_ROUNDS;
   double* pb = prim + (size_t)(blockIdx.x & mask_p) * nprim * 40u;
-  bool todo = in;
-  for (uint64_t mask = __ballot(todo); mask; mask = __ballot(todo)) {
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    const uint32_t lkey = __shfl(key, leader, 64);
-    const bool match = todo && key == lkey;
+  double acc[20];
 #pragma unroll
-    for (int f = 0; f < 10; ++f) {
-      const double s1 = wave_sum(match ? x[f] : 0.0);
-      const double s2 = wave_sum(match ? x[f] * x[f] : 0.0);
-      if ((int)lane_id() == leader) {
-        atomic_add(pb + (size_t)lkey * 20u + 2 * f, s1);
-        atomic_add(pb + (size_t)lkey * 20u + 2 * f + 1, s2);
+  for (int f = 0; f < 20; ++f) acc[f] = 0.0;
+  uint32_t acc_key = NONE;  // uniform over the wave: the key of acc
+  for (uint32_t r = 0; r < PRIM_ROUNDS; ++r) {
+    const uint32_t i = base + r * 64u + lane_id();
+    const bool in = i < m;
+    const uint64_t inmask = __ballot(in);
+    if (!inmask) break;  // lane 0 holds the round's first record from here on
+    const uint32_t key = in ? skey[i] : 0u;
+    double x[10];
+    if (in) {
+      const double2* v = (const double2*)(val + (size_t)sidx[i] * 10u);
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        const double2 t = v[f];
+        x[2 * f] = t.x;
+        x[2 * f + 1] = t.y;
       }
+    } else {
+#pragma unroll
+      for (int f = 0; f < 10; ++f) x[f] = 0.0;
     }
-    todo = todo && !match;
+    const uint32_t k0 = __shfl(key, 0, 64);
+    const bool one_key = __ballot(in && key == k0) == inmask;
+    if (!(one_key && k0 == acc_key) && acc_key != NONE) {
+      prim_flush_run(pb, acc_key, acc, 0);
+#pragma unroll
+      for (int f = 0; f < 20; ++f) acc[f] = 0.0;
+      acc_key = NONE;
+    }
+    if (one_key) {
+      acc_key = k0;
+#pragma unroll
+      for (int f = 0; f < 10; ++f) {
+        acc[2 * f] += x[f];
+        acc[2 * f + 1] += x[f] * x[f];
+      }
+      continue;
+    }
+    // several keys in the round: each run of equal keys on its own
+    bool todo = in;
+    for (uint64_t mask = __ballot(todo); mask; mask = __ballot(todo)) {
+      const int leader = __ffsll((unsigned long long)mask) - 1;
+      const uint32_t lkey = __shfl(key, leader, 64);
+      const bool match = todo && key == lkey;
+      double run[20];
+#pragma unroll
+      for (int f = 0; f < 10; ++f) {
+        run[2 * f] = match ? x[f] : 0.0;
+        run[2 * f + 1] = match ? x[f] * x[f] : 0.0;
+      }
+      prim_flush_run(pb, lkey, run, leader);
+      todo = todo && !match;
+    }
   }
+  if (acc_key != NONE) prim_flush_run(pb, acc_key, acc, 0);
 }
 
 __global__ void k_iota32(uint32_t* a, uint32_t n) {
@@ -3443,9 +3485,9 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->log_sort, sb, c->T.log_key, c->log_skey,
                                                  c->log_idx, c->log_sidx, (int)m, 0,
                                                  (int)c->log_key_bits, c->stream));
-      k_prim_reduce<<<(m + 255) / 256, 256, 0, c->stream>>>(c->log_skey, c->log_sidx,
-                                                           c->T.log_val, m, c->T.prim,
-                                                           c->T.mask_p, c->T.nprim);
+      const uint32_t per_block = 256 * PRIM_ROUNDS;
+      k_prim_reduce<<<(m + per_block - 1) / per_block, 256, 0, c->stream>>>(
+          c->log_skey, c->log_sidx, c->T.log_val, m, c->T.prim, c->T.mask_p, c->T.nprim);
       HIP_TRY(hipGetLastError());
     }
     if (n) HIP_TRY(hipMemsetAsync(c->T.log_n, 0, sizeof(unsigned long long), c->stream));
