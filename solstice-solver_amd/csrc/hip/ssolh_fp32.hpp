@@ -857,8 +857,11 @@ struct Shade32Args {
   unsigned* fetch;  // tail-block counter of a looping launch (for_each_block)
 };
 
+// Four waves per SIMD (at most 128 VGPRs, 28 bytes per lane spilled): on
+// the 10k-heliostat field 25.7-26.3 ms per step where three waves took
+// 26.3-29.6 ms.
 template <bool REFILL, bool LOOP>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_shade32(Shade32Args a) {
   const uint32_t count = *a.count;
   if (LOOP) {  // whole logical blocks, arguments read per block (as k_shade)
