@@ -28,7 +28,9 @@ struct alignas(16) HitRec32 {
 static_assert(sizeof(HitRec32) == 48, "HitRec32 layout");
 
 struct PathState32 {
-  float4* pos;        // offset from the base instance's translation, base (bits)
+  float4* pos;        // start point: offset from the base instance's translation, base
+                      // (bits); read by the path's first shading only (later points
+                      // come from the hit records)
   float4* dir_wl;     // direction, wavelength
   float4* prev_init;  // previous outgoing weights (3), initial flux
   float4* out_in0;    // outgoing weights of the last shaded point (3), its incoming flux
@@ -521,7 +523,9 @@ __device__ __forceinline__ void shade32_block(
     uint4 meta = P.meta[slot];
     uint4 cur = P.cur[slot];
     uint4 misc = P.misc[slot];
-    float4 ps = P.pos[slot];
+    // the start point, shaded after its shadow ray; later points come from
+    // the hit record
+    float4 ps = (meta.w & PF_STAGE_BOUNCE) ? make_float4(0.f, 0.f, 0.f, 0.f) : P.pos[slot];
     float4 dw = P.dir_wl[slot];
     float4 pv = P.prev_init[slot];
     float4 oi = (meta.w & PF_STAGE_BOUNCE) ? P.out_in0[slot] : make_float4(0, 0, 0, 0);
@@ -719,7 +723,6 @@ __device__ __forceinline__ void shade32_block(
       for (int i = 0; i < 6; ++i) fin.g[i] *= fin.factor;
     }
     if (fin.kind == FIN_NONE) {
-      P.pos[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(base));
       P.dir_wl[slot] = make_float4(dir.x, dir.y, dir.z, wl);
       P.prev_init[slot] = pv;
       P.out_in0[slot] = oi;
