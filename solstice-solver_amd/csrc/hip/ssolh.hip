@@ -3236,7 +3236,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   // first reflections, ...), which keeps the waves coherent; larger solves
   // run as many as fit, each slot taking its next realisation in place.
   uint64_t slots64 = path_slots;
-  const size_t per_slot = (fp32 ? 6 * sizeof(float4) + 3 * sizeof(uint4)
+  const size_t per_slot = (fp32 ? 5 * sizeof(float4) + 3 * sizeof(uint4)
                                : sizeof(double2) + 3 * sizeof(double4) + 3 * sizeof(uint4)) +
                           (size_t)npend * 2 * (sizeof(uint32_t) + 10 * (fp32 ? sizeof(float) : sizeof(double))) +
                           (fp32 ? sizeof(RayRec32) + sizeof(HitRec32) : sizeof(RayRec) + sizeof(HitRec)) +
@@ -3286,8 +3286,8 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   std::vector<size_t> sizes;
   if (fp32)
     sizes = {n * sizeof(float4), n * sizeof(float4), n * sizeof(float4), n * sizeof(float4),
-             n * sizeof(float4), n * sizeof(float4), n * sizeof(uint4), n * sizeof(uint4),
-             n * sizeof(uint4), pend_keys, pend_keys, pend_vals, pend_vals};
+             n * sizeof(float4), n * sizeof(uint4), n * sizeof(uint4), n * sizeof(uint4),
+             pend_keys, pend_keys, pend_vals, pend_vals};
   else
     sizes = {n * sizeof(double2), n * sizeof(double4), n * sizeof(double4), n * sizeof(double4),
              n * sizeof(uint4), n * sizeof(uint4), n * sizeof(uint4), pend_keys, pend_keys,
@@ -3302,18 +3302,17 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
     for (size_t i = 0; i < sizes.size(); ++i) { arr[i] = p; p += (sizes[i] + 255) / 256 * 256; }
     if (fp32) {
       c->P32.pos = (float4*)arr[0];
-      c->P32.dir_wl = (float4*)arr[1];
-      c->P32.prev_init = (float4*)arr[2];
-      c->P32.out_in0 = (float4*)arr[3];
-      c->P32.nrm_t = (float4*)arr[4];
-      c->P32.pend_cos = (float4*)arr[5];
-      c->P32.meta = (uint4*)arr[6];
-      c->P32.cur = (uint4*)arr[7];
-      c->P32.misc = (uint4*)arr[8];
-      c->P32.pkey = (uint32_t*)arr[9];
-      c->P32.qkey = (uint32_t*)arr[10];
-      c->P32.pval = (float*)arr[11];
-      c->P32.qval = (float*)arr[12];
+      c->P32.prev_init = (float4*)arr[1];
+      c->P32.out_in0 = (float4*)arr[2];
+      c->P32.nrm_t = (float4*)arr[3];
+      c->P32.pend_cos = (float4*)arr[4];
+      c->P32.meta = (uint4*)arr[5];
+      c->P32.cur = (uint4*)arr[6];
+      c->P32.misc = (uint4*)arr[7];
+      c->P32.pkey = (uint32_t*)arr[8];
+      c->P32.qkey = (uint32_t*)arr[9];
+      c->P32.pval = (float*)arr[10];
+      c->P32.qval = (float*)arr[11];
     } else {
       c->P.cst = (double2*)arr[0];
       c->P.pw = (double4*)arr[1];

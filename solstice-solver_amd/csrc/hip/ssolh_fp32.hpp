@@ -14,7 +14,8 @@ struct alignas(16) RayRec32 {
   uint32_t base;
   float dx, dy, dz;
   float tmin;
-  uint32_t iss_from, prim_from, flags, pad;
+  uint32_t iss_from, prim_from, flags;
+  float wl;             // the path's wavelength (the shade's, not the trace's)
 };
 static_assert(sizeof(RayRec32) == 48, "RayRec32 layout");
 
@@ -31,7 +32,6 @@ struct PathState32 {
   float4* pos;        // start point: offset from the base instance's translation, base
                       // (bits); read by the path's first shading only (later points
                       // come from the hit records)
-  float4* dir_wl;     // direction, wavelength
   float4* prev_init;  // previous outgoing weights (3), initial flux
   float4* out_in0;    // outgoing weights of the last shaded point (3), its incoming flux
   float4* nrm_t;      // normal of the current point, last hit distance
@@ -421,7 +421,6 @@ __device__ void generate32(const DScene& S, const ShapeQ* SQ, const PathState32&
   const uint32_t med = start_medium(S, m, side);
 
   P.pos[slot] = make_float4(off.x, off.y, off.z, __uint_as_float(st.instance));
-  P.dir_wl[slot] = make_float4(dir.x, dir.y, dir.z, wl);
   P.prev_init[slot] = make_float4(w0, w0, w0, w0);
   P.nrm_t[slot] = make_float4(N.x, N.y, N.z, 0);
   P.pend_cos[slot] = make_float4(0, 0, 0, cosf_);
@@ -442,7 +441,7 @@ __device__ void generate32(const DScene& S, const ShapeQ* SQ, const PathState32&
   e.r.iss_from = iss;
   e.r.prim_from = st.prim;
   e.r.flags = (side << RF_SIDE_SHIFT) | RF_DISCARD_VIRTUAL | RF_REVERSED | RF_ANY_HIT;
-  e.r.pad = 0;
+  e.r.wl = wl;
 }
 
 __global__ void __launch_bounds__(BLOCK)
@@ -526,7 +525,6 @@ __device__ __forceinline__ void shade32_block(
     // the start point, shaded after its shadow ray; later points come from
     // the hit record
     float4 ps = (meta.w & PF_STAGE_BOUNCE) ? make_float4(0.f, 0.f, 0.f, 0.f) : P.pos[slot];
-    float4 dw = P.dir_wl[slot];
     float4 pv = P.prev_init[slot];
     float4 oi = (meta.w & PF_STAGE_BOUNCE) ? P.out_in0[slot] : make_float4(0, 0, 0, 0);
     float4 pc = P.pend_cos[slot];
@@ -541,8 +539,10 @@ __device__ __forceinline__ void shade32_block(
     fin.job = misc.w >> 8;
     f3 pos = mkf(ps.x, ps.y, ps.z);
     uint32_t base = __float_as_uint(ps.w);
-    const float wl = dw.w;
-    f3 dir = mkf(dw.x, dw.y, dw.z);
+    // the traced ray carries the path's direction (reversed for the shadow
+    // ray toward the sun) and its wavelength
+    const float wl = r.wl;
+    f3 dir = (meta.w & PF_STAGE_BOUNCE) ? mkf(r.dx, r.dy, r.dz) : mkf(-r.dx, -r.dy, -r.dz);
     f3 N = mkf(0, 0, 0);
     if (!(flags & PF_STAGE_BOUNCE)) {
       const float4 nt = P.nrm_t[slot];
@@ -694,7 +694,7 @@ __device__ __forceinline__ void shade32_block(
               e.r.ox = pos.x; e.r.oy = pos.y; e.r.oz = pos.z;
               e.r.base = base;
               e.r.dx = dir.x; e.r.dy = dir.y; e.r.dz = dir.z;
-              e.r.pad = 0;
+              e.r.wl = wl;
             }
             e.r.tmin = nextafterf(hit_t, FLT_MAX);
           } else {
@@ -702,7 +702,7 @@ __device__ __forceinline__ void shade32_block(
             e.r.base = base;
             e.r.dx = dir.x; e.r.dy = dir.y; e.r.dz = dir.z;
             e.r.tmin = 0;
-            e.r.pad = 0;
+            e.r.wl = wl;
           }
           e.r.iss_from = iss;
           e.r.prim_from = prim;
@@ -723,7 +723,6 @@ __device__ __forceinline__ void shade32_block(
       for (int i = 0; i < 6; ++i) fin.g[i] *= fin.factor;
     }
     if (fin.kind == FIN_NONE) {
-      P.dir_wl[slot] = make_float4(dir.x, dir.y, dir.z, wl);
       P.prev_init[slot] = pv;
       P.out_in0[slot] = oi;
       P.pend_cos[slot] = pc;
