@@ -199,7 +199,8 @@ struct PathState {
                        // initial flux until the first shading, then the
                        // incoming flux of the last virtual point crossed
   double4* nrm;        // normal of the start point (written once per path)
-  double4* pend;       // pending absorbed_by_receivers, extinguished, other
+  double4* pend;       // pending absorbed_by_receivers, extinguished, other (first
+                       // written by the path's first shading, which starts from zeros)
   uint4* st;           // path id lo/hi, rng block, flags
   uint4* sm;           // (instance, shaded shape) index, prim, depth | side << 30,
                        // in medium | out medium << 16
@@ -1544,7 +1545,6 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   P.cst[slot] = make_double2(wl, cosf);
   P.pw[slot] = make_double4(w0, w0, w0, w0);
   P.nrm[slot] = make_double4(N.x, N.y, N.z, 0);
-  P.pend[slot] = make_double4(0, 0, 0, 0);
   P.st[slot] = make_uint4((uint32_t)path, (uint32_t)(path >> 32), rng.block, PF_ACTIVE);
   P.sm[slot] = make_uint4(iss, st.prim, 0u | (side << 30), med | (med << 16));
   P.aux[slot] = make_uint4((uint32_t)inst.sampled, job, 0, 64u);
@@ -1685,7 +1685,7 @@ __device__ __forceinline__ void shade_block(
     // before the first shading, then the incoming flux of the last virtual
     // point the path crossed
     double4 pv = P.pw[slot];
-    double4 pd = P.pend[slot];
+    double4 pd = (st.w & PF_STAGE_BOUNCE) ? P.pend[slot] : make_double4(0, 0, 0, 0);
     // the ray of the emitting point: its origin (the last real point, also
     // past a virtual point, whose ray keeps that origin) and direction
     const RayRec& rr = rays[slot];
