@@ -1548,10 +1548,8 @@ __device__ void generate(const DScene& S, const PathState& P, const SolveParams&
   P.st[slot] = make_uint4((uint32_t)path, (uint32_t)(path >> 32), rng.block, PF_ACTIVE);
   P.sm[slot] = make_uint4(iss, st.prim, 0u | (side << 30), med | (med << 16));
   P.aux[slot] = make_uint4((uint32_t)inst.sampled, job, 0, 64u);
-  for (uint32_t k = 0; k < prm.npend; ++k) {
-    P.pkey[(size_t)k * prm.slots + slot] = NONE;
-    P.qkey[(size_t)k * prm.slots + slot] = NONE;
-  }
+  // the slot's pending keys are all NONE already (cleared with the state,
+  // and by the flush of the slot's previous realisation)
   samp_slot = inst.sampled;
 
   // shadow ray toward the sun (src/ssol_solver.c:325-352)
@@ -2422,6 +2420,22 @@ struct ssolh_context {
   uint64_t nprim = 0;
   uint32_t stack_depth = MAX_STACK;  // traversal stack bound of the scene's 4-wide nodes
   uint32_t lds_stack_cap = 0;        // 16, 24 or 32: LDS stack entries at most (0: MAX_STACK)
+  // The pending keys of every slot are NONE when a solve starts: a solve
+  // that drains leaves them so (each finished path clears its own), so they
+  // are cleared only when the path state's layout changed since the last
+  // solve that drained (state buffer, slots, entries, form) or it came from
+  // the scratch array.
+  struct KeysLayout {
+    void* state;
+    uint64_t slots;
+    uint32_t npend;
+    bool fp32;
+    bool operator==(const KeysLayout& o) const {
+      return state == o.state && slots == o.slots && npend == o.npend && fp32 == o.fp32;
+    }
+  };
+  KeysLayout keys_layout{};
+  bool keys_clean = false;
   float slack_scale = SLAB_SLACK;     // box-test slack of the scenes uploaded next
   // TLAS refit of moved instances: per-object boxes, the TLAS nodes by level
   // (deepest first) and the top copy's sources, all in scene_bufs
@@ -3327,6 +3341,16 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
       c->P.qval = (double*)arr[10];
     }
   }
+  {
+    const ssolh_context::KeysLayout lay{c->state.p, n, npend, fp32};
+    const bool clean = !arena && c->keys_clean && c->keys_layout == lay;
+    c->keys_clean = false;  // until this solve drains
+    c->keys_layout = lay;
+    if (!clean && npend) {
+      HIP_TRY(hipMemsetAsync(fp32 ? c->P32.pkey : c->P.pkey, 0xFF, pend_keys, c->stream));
+      HIP_TRY(hipMemsetAsync(fp32 ? c->P32.qkey : c->P.qkey, 0xFF, pend_keys, c->stream));
+    }
+  }
   if ((r = alloc(c->rays, n * (fp32 ? sizeof(RayRec32) : sizeof(RayRec))))) return r;
   if ((r = alloc(c->hits, n * (fp32 ? sizeof(HitRec32) : sizeof(HitRec))))) return r;
   for (int i = 0; i < 2; ++i) if ((r = alloc(c->lists[i], n * sizeof(uint32_t)))) return r;
@@ -3815,6 +3839,7 @@ int run_solve(ssolh_context* c, uint64_t N, uint64_t max_failed, uint64_t key, u
   c->iterations = iter;
   c->wavefronts = waves;
   c->refilled = slots < N && !chunked;
+  c->keys_clean = !arena;  // drained: every finished path cleared its keys
   return 0;
 }
 

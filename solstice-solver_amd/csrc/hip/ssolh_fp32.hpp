@@ -427,10 +427,7 @@ __device__ void generate32(const DScene& S, const ShapeQ* SQ, const PathState32&
   P.meta[slot] = make_uint4((uint32_t)path, (uint32_t)(path >> 32), rng.block, PF_ACTIVE);
   P.cur[slot] = make_uint4(iss, st.prim, side, (uint32_t)inst.sampled);
   P.misc[slot] = make_uint4(0, med | (med << 16), 0, 64u | (job << 8));
-  for (uint32_t k = 0; k < prm.npend; ++k) {
-    P.pkey[(size_t)k * prm.slots + slot] = NONE;
-    P.qkey[(size_t)k * prm.slots + slot] = NONE;
-  }
+  // pending keys: NONE already (as generate)
   samp_slot = inst.sampled;
 
   e.has = true;
